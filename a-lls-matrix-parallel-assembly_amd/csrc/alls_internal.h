@@ -1,0 +1,163 @@
+// alls_internal.h -- shared definitions of liballs (gfx950 HIP + host C++).
+//
+// Data layout in HBM (one assembly):
+//   hdr   int32 [n_nodes][8]   hdr[a][0] = entries appended to node-row a,
+//                              hdr[a][k] = 1 + index of overflow chunk k (0 = none), k = 1..7
+//   keys  int2  [chunks][C]    (column node b, element id e) of every entry
+//   vals  VT    [chunks][C][B] B = dpn*dpn block of the entry (VT = double, or
+//                              float for FP32-integrated Hex20)
+// Chunk a (a < n_nodes) is node-row a's home chunk; overflow chunks are taken
+// from a bump counter above n_nodes.  Rows are node rows: one entry carries the
+// dpn x dpn block that an element adds to DOF rows a*dpn.. and DOF columns b*dpn..
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ALLS_DIR 8             // chunks addressable per row (home + 7 overflow)
+#define ALLS_SPIN_MAX (1 << 22)
+
+// err[] slots written by kernels, read back by the host after insertion.
+enum {
+    ERR_DIR_OVERFLOW = 0,      // max entries requested by a row beyond ALLS_DIR*C
+    ERR_POOL_EXHAUSTED = 1,
+    ERR_INVALID_MESH = 2,
+    ERR_SPIN_TIMEOUT = 3,
+    ERR_ROW_TOO_BIG = 4,
+    ERR_SLOTS = 8
+};
+// misc[] layout (int32 each, one buffer so a single copy brings them back).
+enum {
+    MISC_ERR = 0,              // ERR_SLOTS entries
+    MISC_OVF_TOP = 8,          // overflow chunks taken
+    MISC_BIG_COUNT = 9,        // rows routed to the wave-per-row kernel
+    MISC_SIZE = 16
+};
+
+struct DevMesh {
+    const double* coords;
+    const int32_t* conn;
+    const uint8_t* types;
+    const double* E_elem;
+    int64_t n_nodes;
+    int64_t n_elems;
+    int32_t dim;
+    int32_t npe;
+    int32_t uniform_type;
+    int32_t _pad;
+    double E, nu, thickness;
+};
+
+struct ListPool {
+    int32_t* hdr;
+    int2* keys;
+    void* vals;
+    int32_t* misc;
+    int64_t n_nodes;           // home chunks
+    int64_t cap_chunks;        // home + overflow capacity
+    int32_t C;                 // entries per chunk
+    int32_t B;                 // values per entry
+};
+
+#ifdef __HIPCC__
+#define ALLS_DEV __device__ __forceinline__
+
+ALLS_DEV int32_t ld_agent(const int32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+ALLS_DEV void st_agent(int32_t* p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+ALLS_DEV int32_t add_agent(int32_t* p, int32_t v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+ALLS_DEV void max_agent(int32_t* p, int32_t v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  Remap so that each
+// XCD works on one contiguous 1/8 of the index space: neighbouring elements
+// (which share rows) then meet in one XCD's L2.
+ALLS_DEV int64_t xcd_block(int64_t bid, int64_t nb) {
+    int64_t q = nb >> 3, r = nb & 7, x = bid & 7, i = bid >> 3;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Reserve n consecutive slots of node-row a; returns the first slot.
+ALLS_DEV int32_t list_reserve(const ListPool& p, int32_t a, int32_t n) {
+    return add_agent(&p.hdr[(int64_t)a * ALLS_DIR], n);
+}
+
+// The reservation [pos, pos+n) owns every chunk k >= 1 whose first slot
+// k*C lies inside it: take that chunk from the pool and publish it.
+ALLS_DEV void list_own_chunks(const ListPool& p, int32_t a, int32_t pos, int32_t n) {
+    int32_t k = (pos + p.C - 1) / p.C;
+    if (k < 1) k = 1;
+    for (; k * p.C < pos + n; ++k) {
+        if (k >= ALLS_DIR) {
+            max_agent(&p.misc[MISC_ERR + ERR_DIR_OVERFLOW], pos + n);
+            return;
+        }
+        int32_t c = add_agent(&p.misc[MISC_OVF_TOP], 1);
+        int32_t tag;
+        if (p.n_nodes + c >= p.cap_chunks) {
+            st_agent(&p.misc[MISC_ERR + ERR_POOL_EXHAUSTED], 1);
+            tag = -1;                              // waiters give up on it
+        } else {
+            tag = c + 1;
+        }
+        st_agent(&p.hdr[(int64_t)a * ALLS_DIR + k], tag);
+    }
+}
+
+// Chunk index holding slot s of row a, or -1.  Waits (bounded) for the
+// owner of an overflow chunk to publish it.
+ALLS_DEV int64_t list_chunk(const ListPool& p, int32_t a, int32_t s) {
+    int32_t k = s / p.C;
+    if (k == 0) return a;
+    if (k >= ALLS_DIR) return -1;
+    const int32_t* d = &p.hdr[(int64_t)a * ALLS_DIR + k];
+    int32_t v = ld_agent(d);
+    for (int32_t it = 0; v == 0; ++it) {
+        if (it >= ALLS_SPIN_MAX) {
+            st_agent(&p.misc[MISC_ERR + ERR_SPIN_TIMEOUT], 1);
+            return -1;
+        }
+        __builtin_amdgcn_s_sleep(2);
+        v = ld_agent(d);
+    }
+    if (v < 0) return -1;
+    return p.n_nodes + (int64_t)(v - 1);
+}
+
+// Chunk of slot s when every chunk is already published (after insertion).
+ALLS_DEV int64_t list_chunk_done(const ListPool& p, const int32_t* dir, int32_t s) {
+    int32_t k = s / p.C;
+    return k == 0 ? -1 : p.n_nodes + (int64_t)(dir[k] - 1);
+}
+#endif  // __HIPCC__
+
+// ----------------------------------------------------------- host side ---
+#ifdef __cplusplus
+#include <string>
+#include <vector>
+
+namespace alls {
+
+// Kernel launchers (defined in the .hip files).
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, hipStream_t s);
+void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, hipStream_t s);
+// Consolidation.  small_group = lanes per row of the register kernel (0 =>
+// every row goes to the wave-per-row LDS kernel).
+void launch_count(const ListPool& p, int small_group, int f32vals, int32_t* u, int32_t* big_list,
+                  hipStream_t s);
+void launch_scan_total(const int32_t* u, int64_t n, int64_t* tiles, int64_t ntiles, hipStream_t s);
+void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dpn, void* row_ptr,
+                       int i64, hipStream_t s);
+void launch_write(const ListPool& p, int small_group, int f32vals, int dpn, const int32_t* big_list,
+                  const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s);
+int64_t scan_tile_count(int64_t n);
+
+}  // namespace alls
+#endif

@@ -1,0 +1,494 @@
+// assemble.cpp -- context, the A-LLS pipeline driver and CSR ownership.
+//
+// Replaces the reference's compareAssemblyMethods A-LLS half (Acc:750-838):
+// that driver allocates heads/pool per run, publishes the pool through
+// __device__ globals and times insert+convert on the legacy stream.  Here the
+// pipeline runs on the context's stream with reusable scratch, checks every
+// HIP call, detects list-directory overflow and retries with larger chunks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "host_internal.h"
+
+namespace alls {
+
+hipError_t DevBuf::ensure(size_t need) {
+    if (need == 0) need = 256;
+    if (need <= bytes) return hipSuccess;
+    release();
+    const size_t nb = need + need / 16 + 256;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return e;
+    }
+    bytes = nb;
+    return hipSuccess;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+hipError_t Workspace::init() {
+    if (!host_misc) {
+        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&host_misc), 64 * sizeof(int64_t), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            host_misc = nullptr;
+            return e;
+        }
+    }
+    if (!events) {
+        for (auto& x : ev) {
+            hipError_t e = hipEventCreate(&x);
+            if (e != hipSuccess) return e;
+        }
+        events = true;
+    }
+    return hipSuccess;
+}
+
+void Workspace::destroy() {
+    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b->release();
+    if (host_misc) (void)hipHostFree(host_misc);
+    host_misc = nullptr;
+    if (events)
+        for (auto& x : ev) (void)hipEventDestroy(x);
+    events = false;
+}
+
+alls_status hip_fail(hipError_t e, const char* what, std::string* err) {
+    if (err) *err = std::string(what) + ": " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? ALLS_E_OOM : ALLS_E_HIP;
+}
+
+#define HIP_TRY(call, what)                                   \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return hip_fail(e_, what, err); \
+    } while (0)
+
+static int kind_nodes(int t) { return t == ALLS_Q4 ? 4 : t == ALLS_T3 ? 3 : t == ALLS_HEX20 ? 20 : -1; }
+
+alls_status classify_mesh(const alls_mesh* m, const alls_opts& o, MeshInfo* mi, std::string* err) {
+    if (!m || !mi) {
+        if (err) *err = "null mesh";
+        return ALLS_E_INVALID;
+    }
+    if (m->n_nodes < 0 || m->n_elems < 0 || m->n_nodes >= (int64_t)INT32_MAX / 3 ||
+        m->n_elems >= (int64_t)INT32_MAX) {
+        if (err) *err = "n_nodes/n_elems out of range";
+        return ALLS_E_INVALID;
+    }
+    if ((m->n_elems > 0 && (!m->conn || !m->coords)) || (m->n_nodes > 0 && !m->coords)) {
+        if (err) *err = "missing coords/conn";
+        return ALLS_E_INVALID;
+    }
+    if (m->mem != ALLS_MEM_HOST && m->mem != ALLS_MEM_DEVICE) {
+        if (err) *err = "bad mem";
+        return ALLS_E_INVALID;
+    }
+    if (m->dim == 3) {
+        if (m->nodes_per_elem != 20 || (m->elem_type == nullptr && m->uniform_type != ALLS_HEX20)) {
+            if (err) *err = "3D meshes must be Hex20 with nodes_per_elem = 20";
+            return ALLS_E_INVALID;
+        }
+        mi->kind = 3;
+        mi->dpn = 3;
+        mi->npe_max = 20;
+        mi->f32 = o.precision == ALLS_PREC_F32_MFMA;
+    } else if (m->dim == 2) {
+        if (m->elem_type) {
+            if (m->nodes_per_elem != 4) {
+                if (err) *err = "mixed 2D meshes need nodes_per_elem = 4 (T3 padded with -1)";
+                return ALLS_E_INVALID;
+            }
+            mi->kind = 2;
+            mi->npe_max = 4;
+        } else if (m->uniform_type == ALLS_Q4) {
+            if (m->nodes_per_elem != 4) {
+                if (err) *err = "Q4 meshes need nodes_per_elem = 4";
+                return ALLS_E_INVALID;
+            }
+            mi->kind = 0;
+            mi->npe_max = 4;
+        } else if (m->uniform_type == ALLS_T3) {
+            if (m->nodes_per_elem < 3) {
+                if (err) *err = "T3 meshes need nodes_per_elem >= 3";
+                return ALLS_E_INVALID;
+            }
+            mi->kind = 1;
+            mi->npe_max = 3;
+        } else {
+            if (err) *err = "2D meshes must be Q4, T3 or mixed";
+            return ALLS_E_INVALID;
+        }
+        mi->dpn = 2;
+        mi->f32 = false;
+    } else {
+        if (err) *err = "dim must be 2 or 3";
+        return ALLS_E_INVALID;
+    }
+    if (m->nu <= -1.0 || m->nu >= 0.5 || (m->dim == 2 && m->nu >= 1.0)) {
+        if (err) *err = "nu out of range";
+        return ALLS_E_INVALID;
+    }
+    mi->entries_bound = m->n_elems * (int64_t)mi->npe_max * mi->npe_max;
+    return ALLS_OK;
+}
+
+static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
+
+alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, DevMesh* dm, hipStream_t s,
+                       std::string* err) {
+    DevMesh d{};
+    d.n_nodes = m->n_nodes;
+    d.n_elems = m->n_elems;
+    d.dim = m->dim;
+    d.npe = m->nodes_per_elem;
+    d.uniform_type = m->uniform_type;
+    d.E = m->E;
+    d.nu = m->nu;
+    d.thickness = m->thickness;
+    const size_t cb = (size_t)m->n_nodes * m->dim * sizeof(double);
+    const size_t kb = (size_t)m->n_elems * m->nodes_per_elem * sizeof(int32_t);
+    const bool host = m->mem == ALLS_MEM_HOST;
+    const hipMemcpyKind kind = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    // Device inputs are used in place when aligned for vector loads.
+    if (!host && aligned16(m->coords)) {
+        d.coords = m->coords;
+    } else if (cb) {
+        HIP_TRY(ws.coords.ensure(cb), "alloc coords");
+        HIP_TRY(hipMemcpyAsync(ws.coords.p, m->coords, cb, kind, s), "copy coords");
+        d.coords = ws.coords.as<double>();
+    }
+    if (!host && aligned16(m->conn)) {
+        d.conn = m->conn;
+    } else if (kb) {
+        HIP_TRY(ws.conn.ensure(kb), "alloc conn");
+        HIP_TRY(hipMemcpyAsync(ws.conn.p, m->conn, kb, kind, s), "copy conn");
+        d.conn = ws.conn.as<int32_t>();
+    }
+    if (m->elem_type) {
+        if (!host) {
+            d.types = m->elem_type;
+        } else if (m->n_elems) {
+            HIP_TRY(ws.types.ensure((size_t)m->n_elems), "alloc types");
+            HIP_TRY(hipMemcpyAsync(ws.types.p, m->elem_type, (size_t)m->n_elems, kind, s), "copy types");
+            d.types = ws.types.as<uint8_t>();
+        }
+    }
+    if (m->E_per_elem) {
+        if (!host) {
+            d.E_elem = m->E_per_elem;
+        } else if (m->n_elems) {
+            HIP_TRY(ws.E.ensure((size_t)m->n_elems * sizeof(double)), "alloc E");
+            HIP_TRY(hipMemcpyAsync(ws.E.p, m->E_per_elem, (size_t)m->n_elems * sizeof(double), kind, s), "copy E");
+            d.E_elem = ws.E.as<double>();
+        }
+    }
+    (void)mi;
+    *dm = d;
+    return ALLS_OK;
+}
+
+alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
+    if (out->cap_rows < n_rows + 1 || !out->row_ptr) {
+        if (out->row_ptr) (void)hipFree(out->row_ptr);
+        out->row_ptr = nullptr;
+        out->cap_rows = 0;
+        if (hipMalloc(&out->row_ptr, (size_t)(n_rows + 1) * sizeof(int64_t)) != hipSuccess) {
+            out->row_ptr = nullptr;
+            (void)hipGetLastError();
+            return ALLS_E_OOM;
+        }
+        out->cap_rows = n_rows + 1;
+    }
+    if (out->cap_nnz < nnz || !out->col_ind || !out->val) {
+        if (out->col_ind) (void)hipFree(out->col_ind);
+        if (out->val) (void)hipFree(out->val);
+        out->col_ind = nullptr;
+        out->val = nullptr;
+        out->cap_nnz = 0;
+        const int64_t n = std::max<int64_t>(nnz, 1);
+        if (hipMalloc(reinterpret_cast<void**>(&out->col_ind), (size_t)n * sizeof(int32_t)) != hipSuccess ||
+            hipMalloc(reinterpret_cast<void**>(&out->val), (size_t)n * sizeof(double)) != hipSuccess) {
+            (void)hipGetLastError();
+            return ALLS_E_OOM;
+        }
+        out->cap_nnz = n;
+    }
+    out->n_rows = n_rows;
+    out->nnz = nnz;
+    out->row_ptr_is_i64 = i64;
+    return ALLS_OK;
+}
+
+void free_csr(alls_csr* out) {
+    if (!out) return;
+    if (out->row_ptr) (void)hipFree(out->row_ptr);
+    if (out->col_ind) (void)hipFree(out->col_ind);
+    if (out->val) (void)hipFree(out->val);
+    std::memset(out, 0, sizeof(*out));
+}
+
+static int default_chunk(int kind) {
+    switch (kind) {
+        case 0: return 16;   // Q4: a structured interior node row gets 4 elements x 4 entries
+        case 1: return 24;   // T3: up to 8 triangles x 3 entries around a grid node
+        case 2: return 24;   // mixed Q4/T3
+        default: return 80;  // Hex20: 20 entries per element; vertex rows 160, edge rows 80
+    }
+}
+static int small_group(int kind) { return kind == 0 ? 16 : kind == 3 ? 0 : 32; }
+
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
+                     hipStream_t s, alls_stats* st, std::string* err) {
+    HIP_TRY(ws.init(), "workspace init");
+    const int dpn = mi.dpn;
+    const int B = dpn * dpn;
+    const size_t vsz = mi.f32 ? sizeof(float) : sizeof(double);
+    const int64_t n_nodes = dm.n_nodes;
+    const int64_t n_rows = n_nodes * dpn;
+    int C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
+    if (C < 4) C = 4;
+    const int G = small_group(mi.kind);
+    const int64_t ntiles = scan_tile_count(n_nodes);
+    int retries = 0;
+    ListPool lp{};
+    int64_t total = 0;
+    for (;;) {
+        const int64_t cap_chunks = n_nodes + (mi.entries_bound + C - 1) / C + 1;
+        HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
+        HIP_TRY(ws.keys.ensure((size_t)cap_chunks * C * sizeof(int2)), "alloc list keys");
+        HIP_TRY(ws.vals.ensure((size_t)cap_chunks * C * B * vsz), "alloc list values");
+        HIP_TRY(ws.misc.ensure(MISC_SIZE * sizeof(int32_t)), "alloc misc");
+        HIP_TRY(ws.u.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc row counts");
+        HIP_TRY(ws.big.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc big-row list");
+        HIP_TRY(ws.tiles.ensure((size_t)(ntiles + 1) * sizeof(int64_t)), "alloc scan tiles");
+        lp.hdr = ws.hdr.as<int32_t>();
+        lp.keys = ws.keys.as<int2>();
+        lp.vals = ws.vals.p;
+        lp.misc = ws.misc.as<int32_t>();
+        lp.n_nodes = n_nodes;
+        lp.cap_chunks = cap_chunks;
+        lp.C = C;
+        lp.B = B;
+
+        HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
+        launch_fill_i32(lp.hdr, n_nodes * ALLS_DIR, 0, s);
+        launch_fill_i32(lp.misc, MISC_SIZE, 0, s);
+        HIP_TRY(hipEventRecord(ws.ev[1], s), "event");
+        if (mi.kind == 3) launch_insert_hex20(dm, lp, mi.f32 ? 1 : 0, s);
+        else launch_insert_2d(dm, lp, mi.kind, o.block_threads, s);
+        HIP_TRY(hipGetLastError(), "launch insert");
+        HIP_TRY(hipEventRecord(ws.ev[2], s), "event");
+        launch_count(lp, G, mi.f32 ? 1 : 0, ws.u.as<int32_t>(), ws.big.as<int32_t>(), s);
+        HIP_TRY(hipGetLastError(), "launch count");
+        HIP_TRY(hipEventRecord(ws.ev[3], s), "event");
+        launch_scan_total(ws.u.as<int32_t>(), n_nodes, ws.tiles.as<int64_t>(), ntiles, s);
+        HIP_TRY(hipGetLastError(), "launch scan");
+        HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
+        HIP_TRY(hipMemcpyAsync(ws.host_misc, lp.misc, MISC_SIZE * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+                "read flags");
+        HIP_TRY(hipMemcpyAsync(ws.host_misc + 32, ws.tiles.as<int64_t>() + ntiles, sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s), "read total");
+        HIP_TRY(hipStreamSynchronize(s), "sync after scan");
+        const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
+        if (f[MISC_ERR + ERR_INVALID_MESH]) {
+            if (err) *err = "mesh has an element with a node id out of range or a wrong element type";
+            return ALLS_E_INVALID;
+        }
+        if (f[MISC_ERR + ERR_SPIN_TIMEOUT]) {
+            if (err) *err = "list chunk publication timed out";
+            return ALLS_E_HIP;
+        }
+        const int32_t need = f[MISC_ERR + ERR_DIR_OVERFLOW];
+        if (need > 0 || f[MISC_ERR + ERR_POOL_EXHAUSTED] || f[MISC_ERR + ERR_ROW_TOO_BIG]) {
+            if (++retries > 6) {
+                if (err) *err = "list directory overflow after retries";
+                return ALLS_E_POOL_OVERFLOW;
+            }
+            int nc = 2 * C;
+            while ((int64_t)nc * ALLS_DIR < need) nc *= 2;
+            C = nc;
+            continue;
+        }
+        total = ws.host_misc[32];
+        if (st) {
+            st->big_rows = f[MISC_BIG_COUNT];
+            st->pool_chunks_used = n_nodes + f[MISC_OVF_TOP];
+            st->pool_chunks_cap = cap_chunks;
+        }
+        break;
+    }
+    const int64_t nnz = total * dpn * dpn;
+    const int i64 = (o.row_ptr_i64 || nnz > (int64_t)INT32_MAX) ? 1 : 0;
+    alls_status rs = ensure_csr(out, n_rows, nnz, i64);
+    if (rs != ALLS_OK) {
+        if (err) *err = "cannot allocate output CSR";
+        return rs;
+    }
+    HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
+    launch_scan_apply(ws.u.as<int32_t>(), n_nodes, ws.tiles.as<int64_t>(), dpn, out->row_ptr, i64, s);
+    HIP_TRY(hipGetLastError(), "launch scan apply");
+    launch_write(lp, G, mi.f32 ? 1 : 0, dpn, ws.big.as<int32_t>(), out->row_ptr, i64, out->col_ind, out->val, s);
+    HIP_TRY(hipGetLastError(), "launch write");
+    HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+    HIP_TRY(hipStreamSynchronize(s), "sync after write");
+    if (st) {
+        st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
+        st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
+        st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
+        st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]) + ev_ms(ws.ev[5], ws.ev[6]) * 0.0;
+        st->ms_write = ev_ms(ws.ev[5], ws.ev[6]);
+        st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);
+        st->n_elems = dm.n_elems;
+        st->n_rows = n_rows;
+        st->nnz = nnz;
+        st->list_entries = mi.entries_bound;
+        st->chunk_slots = C;
+        st->retries = retries;
+        st->scratch_bytes = (int64_t)(ws.hdr.bytes + ws.keys.bytes + ws.vals.bytes);
+    }
+    return ALLS_OK;
+}
+
+}  // namespace alls
+
+using namespace alls;
+
+extern "C" {
+
+const char* alls_status_str(alls_status s) {
+    switch (s) {
+        case ALLS_OK: return "ok";
+        case ALLS_E_INVALID: return "invalid argument or mesh";
+        case ALLS_E_OOM: return "out of memory";
+        case ALLS_E_POOL_OVERFLOW: return "list pool overflow";
+        case ALLS_E_HIP: return "HIP runtime error";
+        case ALLS_E_BUSY: return "admission refused (HBM budget)";
+        case ALLS_E_NO_DEVICE: return "no GPU device";
+        case ALLS_E_UNSUPPORTED: return "unsupported";
+    }
+    return "unknown status";
+}
+
+int alls_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+alls_status alls_ctx_create(alls_ctx** ctx, int device, void* hip_stream) {
+    if (!ctx) return ALLS_E_INVALID;
+    *ctx = nullptr;
+    int n = alls_device_count();
+    if (n <= 0) return ALLS_E_NO_DEVICE;
+    if (device < 0 || device >= n) return ALLS_E_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return ALLS_E_HIP;
+    alls_ctx* c = new alls_ctx();
+    c->device = device;
+    if (hip_stream) {
+        c->stream = static_cast<hipStream_t>(hip_stream);
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return ALLS_E_HIP;
+        }
+        c->own_stream = true;
+    }
+    if (c->ws.init() != hipSuccess) {
+        c->ws.destroy();
+        if (c->own_stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return ALLS_E_HIP;
+    }
+    *ctx = c;
+    return ALLS_OK;
+}
+
+void alls_ctx_destroy(alls_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ws.destroy();
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* alls_ctx_last_error(const alls_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+alls_status alls_last_stats(const alls_ctx* ctx, alls_stats* out) {
+    if (!ctx || !out) return ALLS_E_INVALID;
+    *out = ctx->stats;
+    return ALLS_OK;
+}
+
+alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts, alls_csr* out) {
+    if (!ctx || !mesh || !out) return ALLS_E_INVALID;
+    alls_opts o{};
+    if (opts) o = *opts;
+    ctx->err.clear();
+    (void)hipSetDevice(ctx->device);
+    MeshInfo mi;
+    alls_status s = classify_mesh(mesh, o, &mi, &ctx->err);
+    if (s != ALLS_OK) return s;
+    DevMesh dm;
+    s = stage_mesh(ctx->ws, mesh, mi, &dm, ctx->stream, &ctx->err);
+    if (s != ALLS_OK) return s;
+    alls_stats st{};
+    s = run_alls(ctx->ws, dm, mi, o, out, ctx->stream, &st, &ctx->err);
+    if (s == ALLS_OK) ctx->stats = st;
+    return s;
+}
+
+alls_status alls_csr_download(alls_ctx* ctx, const alls_csr* csr, int64_t* row_ptr, int32_t* col_ind, double* val) {
+    if (!ctx || !csr) return ALLS_E_INVALID;
+    (void)hipSetDevice(ctx->device);
+    std::string* err = &ctx->err;
+    HIP_TRY(hipStreamSynchronize(ctx->stream), "sync");
+    if (row_ptr && csr->row_ptr) {
+        if (csr->row_ptr_is_i64) {
+            HIP_TRY(hipMemcpy(row_ptr, csr->row_ptr, (size_t)(csr->n_rows + 1) * sizeof(int64_t), hipMemcpyDeviceToHost),
+                    "download row_ptr");
+        } else {
+            std::vector<int32_t> tmp((size_t)csr->n_rows + 1);
+            HIP_TRY(hipMemcpy(tmp.data(), csr->row_ptr, tmp.size() * sizeof(int32_t), hipMemcpyDeviceToHost),
+                    "download row_ptr");
+            for (size_t i = 0; i < tmp.size(); ++i) row_ptr[i] = tmp[i];
+        }
+    }
+    if (col_ind && csr->col_ind && csr->nnz)
+        HIP_TRY(hipMemcpy(col_ind, csr->col_ind, (size_t)csr->nnz * sizeof(int32_t), hipMemcpyDeviceToHost),
+                "download col_ind");
+    if (val && csr->val && csr->nnz)
+        HIP_TRY(hipMemcpy(val, csr->val, (size_t)csr->nnz * sizeof(double), hipMemcpyDeviceToHost), "download val");
+    return ALLS_OK;
+}
+
+void alls_csr_free(alls_ctx* ctx, alls_csr* csr) {
+    if (ctx) (void)hipSetDevice(ctx->device);
+    free_csr(csr);
+}
+
+}  // extern "C"

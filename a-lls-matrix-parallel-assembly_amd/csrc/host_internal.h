@@ -1,0 +1,64 @@
+// host_internal.h -- host-side (C++) internals of liballs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "alls.h"
+#include "alls_internal.h"
+
+namespace alls {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need);
+    void release();
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// What the pipeline needs to know about a mesh before launching.
+struct MeshInfo {
+    int kind = 0;              // 0 Q4, 1 T3, 2 mixed Q4/T3, 3 Hex20
+    int dpn = 2;               // DOF per node (= dim)
+    int npe_max = 4;           // nodes per element (max)
+    int64_t entries_bound = 0; // upper bound of list entries (sum of npe^2)
+    bool f32 = false;          // FP32-MFMA Hex20 integration
+};
+
+// Per-stream scratch: list headers, pool, consolidation arrays and a device
+// copy of host meshes.  A context owns one; the server owns one per slot.
+struct Workspace {
+    DevBuf hdr, keys, vals, misc, u, big, tiles;
+    DevBuf coords, conn, types, E;
+    int64_t* host_misc = nullptr;   // pinned readback: misc ints + scan total
+    hipEvent_t ev[8] = {};
+    bool events = false;
+    hipError_t init();
+    void destroy();
+};
+
+alls_status classify_mesh(const alls_mesh* m, const alls_opts& o, MeshInfo* mi, std::string* err);
+// Device view of the mesh; host arrays are copied into the workspace.
+alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, DevMesh* dm, hipStream_t s,
+                       std::string* err);
+// Full A-LLS pipeline on `s`: insert, count, scan, write.  Synchronises `s`.
+alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
+                     hipStream_t s, alls_stats* st, std::string* err);
+// Makes sure `out` can hold n_rows rows and nnz entries.
+alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64);
+void free_csr(alls_csr* out);
+alls_status hip_fail(hipError_t e, const char* what, std::string* err);
+
+}  // namespace alls
+
+struct alls_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    alls_stats stats{};
+    alls::Workspace ws;
+};

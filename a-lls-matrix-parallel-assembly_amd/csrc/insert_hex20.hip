@@ -1,0 +1,236 @@
+// insert_hex20.hip -- Hex20 3D elasticity: MFMA element integration fused
+// with A-LLS list insertion (gfx950).
+//
+// Replaces computeElementStiffnessHex20 (Hex:130-195) and
+// assembleGlobalStiffnessKernelALLSFixedArray (Hex:208-257).
+//
+// For isotropic elasticity the 3x3 block of node pair (I,J) is
+//   K_IJ = E * (lam * M_IJ + mu * M_IJ^T + mu * tr(M_IJ) * I3),
+//   M[(I,p),(J,q)] = sum_g w_g * dN_I/dx_p(g) * dN_J/dx_q(g),
+// the same sum the reference writes out term by term (Hex:173-191).  M is
+// the 60x60 product X^T (W X) with X the 27x60 matrix of Gauss-point
+// gradients, so one wavefront per element forms X in LDS and contracts it
+// on the matrix cores: v_mfma_f64_16x16x4f64 (FP64 path) or
+// v_mfma_f32_16x16x4f32 (FP32 path), over the 10 upper-triangle 16x16 tiles
+// of the symmetric 64x64 (padded) product, 7 k-steps of 4 Gauss points.
+// K is then formed from M in LDS and every row node's 20 entries are
+// appended to its list with one reservation atomic.
+#include "alls_internal.h"
+
+namespace {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Gauss rule exactly as the reference writes it (Hex:45-46).
+__constant__ double HGP[3] = {-0.774596669241483, 0.0, 0.774596669241483};
+__constant__ double HGW[3] = {0.555555555555556, 0.888888888888889, 0.555555555555556};
+// Natural coordinates of the 20 nodes (Hex:97-99).
+__constant__ double HXI[20] = {-1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1, -1};
+__constant__ double HETA[20] = {-1, -1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1};
+__constant__ double HZETA[20] = {-1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1, 0, 0, 0, 0};
+
+constexpr int KS = 28;   // Gauss rows padded 27 -> 28 (7 k-steps of 4)
+constexpr int XS = 64;   // gradient columns padded 60 -> 64
+constexpr int MS = 61;   // row stride of M in LDS
+
+// dN/d(xi,eta,zeta) of node n at a Gauss point (Hex:95-128).
+__device__ __forceinline__ void hex20_dshape(int n, double xi, double eta, double zeta, double& dx, double& de,
+                                             double& dz) {
+    const double a = HXI[n], b = HETA[n], c = HZETA[n];
+    if (n < 8) {
+        dx = 0.125 * a * (1.0 + b * eta) * (1.0 + c * zeta) * (2.0 * a * xi + b * eta + c * zeta - 1.0);
+        de = 0.125 * b * (1.0 + a * xi) * (1.0 + c * zeta) * (a * xi + 2.0 * b * eta + c * zeta - 1.0);
+        dz = 0.125 * c * (1.0 + a * xi) * (1.0 + b * eta) * (a * xi + b * eta + 2.0 * c * zeta - 1.0);
+    } else if (n >= 16) {
+        dx = 0.25 * a * (1.0 + b * eta) * (1.0 - zeta * zeta);
+        de = 0.25 * b * (1.0 + a * xi) * (1.0 - zeta * zeta);
+        dz = -0.5 * zeta * (1.0 + a * xi) * (1.0 + b * eta);
+    } else if ((n & 1) == 0) {  // 8, 10, 12, 14: xi_n = 0
+        dx = -0.5 * xi * (1.0 + b * eta) * (1.0 + c * zeta);
+        de = 0.25 * b * (1.0 - xi * xi) * (1.0 + c * zeta);
+        dz = 0.25 * c * (1.0 - xi * xi) * (1.0 + b * eta);
+    } else {                    // 9, 11, 13, 15: eta_n = 0
+        dx = 0.25 * a * (1.0 - eta * eta) * (1.0 + c * zeta);
+        de = -0.5 * eta * (1.0 + a * xi) * (1.0 + c * zeta);
+        dz = 0.25 * c * (1.0 + a * xi) * (1.0 - eta * eta);
+    }
+}
+
+__device__ __forceinline__ f64x4 mfma(double a, double b, f64x4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T> struct Acc;
+template <> struct Acc<double> { typedef f64x4 type; };
+template <> struct Acc<float> { typedef f32x4 type; };
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_insert_hex20(DevMesh m, ListPool p) {
+    constexpr int LDS_T = (2 * KS * XS > 60 * MS) ? 2 * KS * XS : 60 * MS;
+    __shared__ T lds[LDS_T];
+    __shared__ double xyz[60];
+    __shared__ int32_t nodes[20];
+    __shared__ int32_t posv[20];
+    __shared__ int32_t bad;
+
+    const int64_t e = xcd_block(blockIdx.x, gridDim.x);
+    if (e >= m.n_elems) return;
+    const int lane = threadIdx.x;
+    if (lane == 0) bad = 0;
+    __syncthreads();
+    if (lane < 20) {
+        const int32_t n = m.conn[e * m.npe + lane];
+        if (n < 0 || n >= m.n_nodes) {
+            bad = 1;
+            nodes[lane] = 0;
+        } else {
+            nodes[lane] = n;
+            xyz[lane * 3 + 0] = m.coords[3 * (int64_t)n + 0];
+            xyz[lane * 3 + 1] = m.coords[3 * (int64_t)n + 1];
+            xyz[lane * 3 + 2] = m.coords[3 * (int64_t)n + 2];
+        }
+    }
+    __syncthreads();
+    if (bad || (m.types && m.types[e] != 2)) {
+        if (lane == 0) st_agent(&p.misc[MISC_ERR + ERR_INVALID_MESH], 1);
+        return;
+    }
+
+    // Gauss-point gradients: lane g < 27 fills row g of X and W*X.
+    T* X = lds;
+    T* Y = lds + KS * XS;
+    if (lane < KS) {
+        if (lane < 27) {
+            const int gi = lane / 9, gj = (lane / 3) % 3, gk = lane % 3;
+            const double xi = HGP[gi], eta = HGP[gj], zeta = HGP[gk];
+            const double weight = HGW[gi] * HGW[gj] * HGW[gk];
+            double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int n = 0; n < 20; ++n) {
+                double dx, de, dz;
+                hex20_dshape(n, xi, eta, zeta, dx, de, dz);
+                const double px = xyz[3 * n], py = xyz[3 * n + 1], pz = xyz[3 * n + 2];
+                J[0] += dx * px; J[1] += dx * py; J[2] += dx * pz;
+                J[3] += de * px; J[4] += de * py; J[5] += de * pz;
+                J[6] += dz * px; J[7] += dz * py; J[8] += dz * pz;
+            }
+            double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                         J[2] * (J[3] * J[7] - J[4] * J[6]);
+            if (fabs(det) < 1e-15) det = 1e-15;  // Hex:156
+            const double r = 1.0 / det;
+            const double i0 = (J[4] * J[8] - J[5] * J[7]) * r, i1 = (J[2] * J[7] - J[1] * J[8]) * r;
+            const double i2 = (J[1] * J[5] - J[2] * J[4]) * r, i3 = (J[5] * J[6] - J[3] * J[8]) * r;
+            const double i4 = (J[0] * J[8] - J[2] * J[6]) * r, i5 = (J[2] * J[3] - J[0] * J[5]) * r;
+            const double i6 = (J[3] * J[7] - J[4] * J[6]) * r, i7 = (J[1] * J[6] - J[0] * J[7]) * r;
+            const double i8 = (J[0] * J[4] - J[1] * J[3]) * r;
+            const double w = weight * det;
+            T* xr = X + lane * XS;
+            T* yr = Y + lane * XS;
+            for (int n = 0; n < 20; ++n) {
+                double dx, de, dz;
+                hex20_dshape(n, xi, eta, zeta, dx, de, dz);
+                const double gx = i0 * dx + i1 * de + i2 * dz;
+                const double gy = i3 * dx + i4 * de + i5 * dz;
+                const double gz = i6 * dx + i7 * de + i8 * dz;
+                xr[3 * n + 0] = (T)gx; xr[3 * n + 1] = (T)gy; xr[3 * n + 2] = (T)gz;
+                yr[3 * n + 0] = (T)(w * gx); yr[3 * n + 1] = (T)(w * gy); yr[3 * n + 2] = (T)(w * gz);
+            }
+            for (int c = 60; c < XS; ++c) xr[c] = yr[c] = (T)0;
+        } else {
+            for (int c = 0; c < XS; ++c) X[lane * XS + c] = Y[lane * XS + c] = (T)0;
+        }
+    }
+    __syncthreads();
+
+    // M = X^T Y on the matrix cores: tiles (tr <= tc) of 16x16, A = X^T, B = Y.
+    typedef typename Acc<T>::type acc_t;
+    acc_t acc[10];
+#pragma unroll
+    for (int t = 0; t < 10; ++t) acc[t] = acc_t{0, 0, 0, 0};
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int s = 0; s < KS / 4; ++s) {
+        T av[4], bv[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            av[t] = X[(4 * s + lk) * XS + 16 * t + li];
+            bv[t] = Y[(4 * s + lk) * XS + 16 * t + li];
+        }
+        int t = 0;
+#pragma unroll
+        for (int tr = 0; tr < 4; ++tr)
+#pragma unroll
+            for (int tc = tr; tc < 4; ++tc) {
+                acc[t] = mfma(av[tr], bv[tc], acc[t]);
+                ++t;
+            }
+    }
+    __syncthreads();   // X/Y are overwritten by M below
+
+    // Scatter the tiles into M (row stride MS), mirroring the upper triangle.
+    T* M = lds;
+    {
+        int t = 0;
+#pragma unroll
+        for (int tr = 0; tr < 4; ++tr)
+#pragma unroll
+            for (int tc = tr; tc < 4; ++tc) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 16 * tr + 4 * lk + q, c = 16 * tc + li;
+                    if (r < 60 && c < 60) {
+                        M[r * MS + c] = acc[t][q];
+                        if (tr != tc) M[c * MS + r] = acc[t][q];
+                    }
+                }
+                ++t;
+            }
+    }
+    // Reserve 20 slots in each of the element's 20 node rows.
+    if (lane < 20) {
+        const int32_t pos = list_reserve(p, nodes[lane], 20);
+        list_own_chunks(p, nodes[lane], pos, 20);
+        posv[lane] = pos;
+    }
+    __syncthreads();
+
+    const double E = m.E_elem ? m.E_elem[e] : m.E;
+    const double nu = m.nu;
+    const T lam = (T)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+    const T mu = (T)(E / (2.0 * (1.0 + nu)));
+    for (int pr = lane; pr < 400; pr += 64) {
+        const int I = pr / 20, Jn = pr % 20;
+        const int32_t s = posv[I] + Jn;
+        const int64_t ch = list_chunk(p, nodes[I], s);
+        if (ch < 0) continue;
+        const int64_t idx = ch * p.C + (s % p.C);
+        T mb[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) mb[r][c] = M[(3 * I + r) * MS + 3 * Jn + c];
+        const T tr = mb[0][0] + mb[1][1] + mb[2][2];
+        T* dst = static_cast<T*>(p.vals) + idx * 9;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dst[r * 3 + c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
+        p.keys[idx] = make_int2(nodes[Jn], (int32_t)e);
+    }
+}
+
+}  // namespace
+
+namespace alls {
+
+void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, hipStream_t s) {
+    if (m.n_elems <= 0) return;
+    if (f32)
+        hipLaunchKernelGGL(k_insert_hex20<float>, dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p);
+    else
+        hipLaunchKernelGGL(k_insert_hex20<double>, dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p);
+}
+
+}  // namespace alls

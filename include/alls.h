@@ -1,0 +1,263 @@
+/*
+ * alls.h -- C ABI of liballs.so, MI355X (gfx950) A-LLS finite-element
+ * stiffness assembly.
+ *
+ * Inputs are node coordinates, element connectivity and material constants.
+ * The output is a CSR triple (row_ptr, col_ind, val).  Its pattern is
+ * bit-identical to the reference's host std::set pattern, and its values
+ * match the reference's serial element-order assembly.
+ *
+ * The reference (zhou1533/A-LLS-Matrix-Parallel-Assembly) has no library or
+ * FFI.  Each entry point below names the reference code it replaces.  Keys:
+ *   Acc = ALLS_Quad2D_Assembly_Accuracy_Analysis.cu
+ *   T3  = ALLS_T3_Unstructed_Elements_Assemble_Performance.cu
+ *   Mix = ALLS_Q4_T3_Mixed_Elements_Assembly_Performance.cu
+ *   Hex = ALLS_Hex20_3D_Assembly_Performance.cu
+ *
+ * Conventions:
+ *  - Errors are status codes; the library never exits (the reference's
+ *    CHECK_CUDA_ERROR calls exit(), Acc:15-23).
+ *  - No global state.  One context per host thread; all work is ordered on
+ *    the context's HIP stream.  The reference keeps its pool pointers in
+ *    __device__ globals (Acc:384-385, 790-791), which makes it non-reentrant.
+ *  - Output CSR buffers are device memory owned by the library.  Release them
+ *    with alls_csr_free.  Passing an alls_csr that already holds large-enough
+ *    buffers reuses them.
+ */
+#ifndef ALLS_H
+#define ALLS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ALLS_OK = 0,
+    ALLS_E_INVALID = 1,       /* bad argument or malformed mesh (node id out of range, ...) */
+    ALLS_E_OOM = 2,           /* device or host allocation failed */
+    ALLS_E_POOL_OVERFLOW = 3, /* a row outgrew the list directory after all retries */
+    ALLS_E_HIP = 4,           /* HIP runtime error (message in alls_ctx_last_error) */
+    ALLS_E_BUSY = 5,          /* admission refused: request exceeds the HBM budget */
+    ALLS_E_NO_DEVICE = 6,     /* no usable GPU */
+    ALLS_E_UNSUPPORTED = 7
+} alls_status;
+
+/* Element kinds.  Q4=0 and T3=1 are the reference's ElementType values (T3:37-40). */
+typedef enum { ALLS_Q4 = 0, ALLS_T3 = 1, ALLS_HEX20 = 2 } alls_elem_type;
+typedef enum { ALLS_MEM_HOST = 0, ALLS_MEM_DEVICE = 1 } alls_mem;
+/* Precision of Hex20 element integration.  Both forms accumulate and
+ * consolidate in FP64.  F32_MFMA contracts the Gauss-point gradients on FP32
+ * matrix cores and stores FP32 element contributions. */
+typedef enum { ALLS_PREC_F64 = 0, ALLS_PREC_F32_MFMA = 1 } alls_precision;
+
+typedef struct alls_ctx alls_ctx;
+typedef struct alls_pattern alls_pattern;
+typedef struct alls_coloring alls_coloring;
+typedef struct alls_server alls_server;
+
+/* Mesh in structure-of-arrays form.  It replaces the reference's Element AoS,
+ * which copies node coordinates into every element (Acc:51-56, T3:53-60,
+ * Hex:49-55), and its compile-time material constants (Acc:39-41, Hex:43-44). */
+typedef struct {
+    int32_t dim;              /* 2 (Q4/T3/mixed) or 3 (Hex20); also DOF per node */
+    int32_t nodes_per_elem;   /* stride of conn: 4 (Q4, or mixed Q4/T3), 3 (T3), 20 (Hex20) */
+    int64_t n_nodes;
+    int64_t n_elems;
+    const double*  coords;    /* [n_nodes*dim], interleaved x,y(,z) */
+    const int32_t* conn;      /* [n_elems*nodes_per_elem]; unused trailing slots -1 (T3 in a mixed mesh) */
+    const uint8_t* elem_type; /* [n_elems] alls_elem_type, or NULL => uniform_type */
+    int32_t uniform_type;
+    int32_t mem;              /* alls_mem of coords, conn, elem_type and E_per_elem */
+    double E;                 /* Young's modulus, used when E_per_elem == NULL */
+    const double* E_per_elem; /* [n_elems] or NULL */
+    double nu;                /* Poisson ratio */
+    double thickness;         /* plane-stress thickness (2D) */
+} alls_mesh;
+
+typedef struct {
+    int32_t precision;        /* alls_precision (Hex20 only) */
+    int32_t chunk_slots;      /* entries per list-pool chunk; 0 => tuned default */
+    int32_t block_threads;    /* threads per block of the insert kernel; 0 => tuned default */
+    int32_t row_ptr_i64;      /* 1 => always int64 row_ptr; otherwise int64 only when nnz >= 2^31 */
+    int32_t timing;           /* 1 => time each stage with hipEvents (alls_last_stats) */
+    int32_t reserved[3];
+} alls_opts;
+
+/* CSR triple in device memory.  Zero-initialise before first use. */
+typedef struct {
+    int64_t n_rows;
+    int64_t nnz;
+    int32_t row_ptr_is_i64;   /* row_ptr element type: int32 (0) or int64 (1) */
+    int32_t _pad;
+    void*    row_ptr;         /* device [n_rows+1] */
+    int32_t* col_ind;         /* device [nnz] */
+    double*  val;             /* device [nnz] */
+    int64_t cap_rows;         /* capacity of row_ptr in rows (library bookkeeping) */
+    int64_t cap_nnz;          /* capacity of col_ind/val in entries */
+} alls_csr;
+
+/* Per-call statistics of the last assembly on a context. */
+typedef struct {
+    double ms_total;          /* device time of the whole pipeline (hipEvents) */
+    double ms_init;           /* list header reset */
+    double ms_insert;         /* element integration + list insertion */
+    double ms_count;          /* per-row sort + unique count */
+    double ms_scan;           /* device prefix scan -> row_ptr */
+    double ms_write;          /* per-row sort + merge -> col_ind, val */
+    int64_t n_elems;
+    int64_t n_rows;
+    int64_t nnz;
+    int64_t list_entries;     /* entries appended to the row lists */
+    int64_t pool_chunks_used;
+    int64_t pool_chunks_cap;
+    int64_t scratch_bytes;    /* device bytes of list headers + pool */
+    int32_t chunk_slots;
+    int32_t retries;          /* directory-overflow retries taken */
+    int32_t n_colors;         /* colored path only */
+    int32_t big_rows;         /* rows consolidated by the LDS (wave-per-row) kernel */
+} alls_stats;
+
+/* ------------------------------------------------------------ context --- */
+const char* alls_status_str(alls_status s);
+/* Creates a context on `device`.  `hip_stream` is a hipStream_t, or NULL for
+ * a stream the context creates and owns. */
+alls_status alls_ctx_create(alls_ctx** ctx, int device, void* hip_stream);
+void        alls_ctx_destroy(alls_ctx* ctx);
+const char* alls_ctx_last_error(const alls_ctx* ctx);
+alls_status alls_last_stats(const alls_ctx* ctx, alls_stats* out);
+/* Number of visible GPUs (0 when none). */
+int         alls_device_count(void);
+
+/* ----------------------------------------------------------- assembly --- */
+/* Plain A-LLS assembly: element integration, atomic insertion into row-wise
+ * linked lists, and device consolidation into CSR.
+ * Replaces, as one call:
+ *   buildCSRStructure                   Acc:605-645 / T3:458-482 / Mix:488-512 / Hex:364-392 (host std::set pattern)
+ *   assembleGlobalStiffnessKernelALLS   Acc:394-468 / T3:227-290 / Mix:248-312 (+ FixedArray, Hex:208-257)
+ *   convertALLStoCSRKernel              Acc:471-501 / T3:292-314 / Mix:314-339 / Hex:259-278
+ * and returns the values the reference's serial assembleGlobalStiffnessHost
+ * (Acc:329-380) computes on that pattern. */
+alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts, alls_csr* out);
+
+/* Copies a CSR to host memory.  row_ptr is always returned as int64.  Any
+ * of the three destinations may be NULL. */
+alls_status alls_csr_download(alls_ctx* ctx, const alls_csr* csr, int64_t* row_ptr,
+                              int32_t* col_ind, double* val);
+void        alls_csr_free(alls_ctx* ctx, alls_csr* csr);
+
+/* ------------------------------------------------- cached sparsity pattern */
+/* Runs alls_assemble and keeps a device copy of the mesh, its CSR pattern and
+ * an element -> CSR-slot scatter map, so later value-only assemblies skip
+ * list construction.  The reference rebuilds everything for each run and
+ * keeps its host pattern only (Acc:1014-1016). */
+alls_status alls_pattern_create(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts,
+                                alls_pattern** pat, alls_csr* out);
+/* Value-only re-assembly on a cached pattern with a new per-element E field
+ * (E_per_elem NULL => the mesh's scalar E).  col_ind/row_ptr are rewritten
+ * unchanged; val is recomputed with FP64 atomics through the scatter map. */
+alls_status alls_assemble_values(alls_ctx* ctx, const alls_pattern* pat, const double* E_per_elem,
+                                 int32_t mem, alls_csr* out);
+int64_t     alls_pattern_nnz(const alls_pattern* pat);
+void        alls_pattern_free(alls_ctx* ctx, alls_pattern* pat);
+
+/* ------------------------------------------------------- GPU coloring --- */
+/* Element colouring through shared nodes on the GPU (Jones-Plassmann-Luby
+ * with first-fit colours).  Replaces the host greedy colourers colorElements
+ * (Acc:648-747, T3:484-523) and colorElementsHex20 (Hex:394-417). */
+alls_status alls_color(alls_ctx* ctx, const alls_pattern* pat, alls_coloring** col);
+/* Wraps a caller-provided colouring (host int32 [n_elems]); it is checked for
+ * conflicts on the device. */
+alls_status alls_coloring_from_host(alls_ctx* ctx, const alls_pattern* pat, const int32_t* colors,
+                                    alls_coloring** col);
+int32_t     alls_coloring_count(const alls_coloring* col);
+alls_status alls_coloring_download(alls_ctx* ctx, const alls_coloring* col, int32_t* colors);
+void        alls_coloring_free(alls_ctx* ctx, alls_coloring* col);
+/* Coloured assembly on a cached pattern: one full-grid launch per colour with
+ * plain (non-atomic) "+=" into binary-searched CSR slots.  Replaces
+ * assembleGlobalStiffnessKernelColored[NoAtomic] (Acc:504-572, T3:316-363,
+ * Mix:341-390, Hex:281-321), which run all colours at once in one block each. */
+alls_status alls_assemble_colored(alls_ctx* ctx, const alls_pattern* pat, const alls_coloring* col,
+                                  const double* E_per_elem, int32_t mem, alls_csr* out);
+
+/* ------------------------------------------------------------ batching --- */
+/* Assembles n meshes of uneven size in one hipGraph-captured launch
+ * sequence.  Every stage is a single persistent launch over all meshes. */
+alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshes,
+                                const alls_opts* opts, alls_csr* outs);
+
+/* ------------------------------------------------------------- serving --- */
+/* Continuous-batching assembly service on one GPU.  Requests are admitted
+ * while their HBM footprint fits the budget, run concurrently on up to
+ * n_streams streams, and reuse cached patterns when mesh_key repeats. */
+typedef struct {
+    int32_t device;
+    int32_t n_streams;        /* concurrent in-flight requests (<= 4 hardware queues by default) */
+    int64_t hbm_budget_bytes; /* admission budget; 0 => 80% of free HBM */
+    int32_t pattern_cache_entries;
+    int32_t reserved;
+} alls_server_config;
+
+typedef struct {
+    double queue_ms;          /* submit -> admitted */
+    double run_ms;            /* admitted -> done (device work of this request) */
+    double latency_ms;        /* submit -> done */
+    int64_t nnz;
+    int32_t cache_hit;        /* 1 => value-only assembly on a cached pattern */
+    int32_t status;           /* alls_status of the request */
+} alls_request_stats;
+
+alls_status alls_server_create(const alls_server_config* cfg, alls_server** srv);
+/* Enqueues a request; the mesh arrays must stay valid until it completes.
+ * mesh_key != 0 enables the pattern cache for this mesh.  Returns ALLS_E_BUSY
+ * if the request alone exceeds the whole budget. */
+alls_status alls_server_submit(alls_server* srv, const alls_mesh* mesh, uint64_t mesh_key,
+                               const alls_opts* opts, int64_t* ticket);
+/* Blocks until the request completes.  Its CSR moves into *out (the caller
+ * frees it with alls_server_release). */
+alls_status alls_server_wait(alls_server* srv, int64_t ticket, alls_csr* out, alls_request_stats* st);
+void        alls_server_release(alls_server* srv, alls_csr* csr);
+void        alls_server_destroy(alls_server* srv);
+
+/* --------------------------------------------------- synthetic meshes --- */
+/* Host-side generators of the benchmark meshes (own counter-based RNG).
+ * They replace generateMesh (Acc:575-603), MeshGenerator (T3:366-456,
+ * Mix:393-486) and generateMeshHex20 (Hex:324-362), adding node jitter,
+ * random diagonals, node-id permutation and lognormal E. */
+typedef enum {
+    ALLS_GEN_Q4_GRID = 0,     /* nx*ny Q4 cells on the unit square */
+    ALLS_GEN_T3_GRID = 1,     /* every cell split into 2 T3 */
+    ALLS_GEN_MIXED_GRID = 2,  /* split_fraction of cells split into 2 T3, rest Q4 */
+    ALLS_GEN_HEX20_BRICK = 3  /* nx*ny*nz Hex20 on the unit cube, compact node numbering */
+} alls_gen_kind;
+
+typedef struct {
+    int32_t kind;             /* alls_gen_kind */
+    int32_t nx, ny, nz;
+    double jitter;            /* interior-node jitter, fraction of the cell size (uniform +-) */
+    uint64_t seed;
+    int32_t random_diagonal;  /* T3/mixed: 1 => random split diagonal, 0 => n0-n2 as the reference */
+    int32_t permute_nodes;    /* 1 => random permutation of node ids */
+    double split_fraction;    /* mixed: probability that a cell is split */
+    double E;                 /* scalar E (used when E_sigma == 0) */
+    double E_sigma;           /* > 0 => per-element E = exp(E_sigma * N(0,1)) */
+    double nu;
+    double thickness;
+} alls_gen_spec;
+
+typedef struct {
+    alls_mesh mesh;           /* host arrays owned by the library */
+    int32_t* structured_color;/* [n_elems] cell-parity colouring (4 colours in 2D, 8 in 3D) */
+    int32_t n_structured_colors;
+    int32_t _pad;
+} alls_host_mesh;
+
+alls_status alls_gen_mesh(const alls_gen_spec* spec, alls_host_mesh* out);
+void        alls_host_mesh_free(alls_host_mesh* m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALLS_H */
