@@ -1,0 +1,400 @@
+"""ctypes binding of liballs.so (the C ABI in include/alls.h).
+
+This is the Python face of the library used by tests, bench.py and the
+smoke check.  It loads the in-tree liballs.so and raises if it is missing:
+there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liballs.so")
+
+OK, E_INVALID, E_OOM, E_POOL_OVERFLOW, E_HIP, E_BUSY, E_NO_DEVICE, E_UNSUPPORTED = range(8)
+Q4, T3, HEX20 = 0, 1, 2
+MEM_HOST, MEM_DEVICE = 0, 1
+PREC_F64, PREC_F32_MFMA = 0, 1
+GEN_Q4, GEN_T3, GEN_MIXED, GEN_HEX20 = 0, 1, 2, 3
+
+
+class Mesh(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("nodes_per_elem", C.c_int32),
+        ("n_nodes", C.c_int64), ("n_elems", C.c_int64),
+        ("coords", C.c_void_p), ("conn", C.c_void_p), ("elem_type", C.c_void_p),
+        ("uniform_type", C.c_int32), ("mem", C.c_int32),
+        ("E", C.c_double), ("E_per_elem", C.c_void_p),
+        ("nu", C.c_double), ("thickness", C.c_double),
+    ]
+
+
+class Opts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("chunk_slots", C.c_int32), ("block_threads", C.c_int32),
+        ("row_ptr_i64", C.c_int32), ("timing", C.c_int32), ("reserved", C.c_int32 * 3),
+    ]
+
+
+class Csr(C.Structure):
+    _fields_ = [
+        ("n_rows", C.c_int64), ("nnz", C.c_int64), ("row_ptr_is_i64", C.c_int32), ("_pad", C.c_int32),
+        ("row_ptr", C.c_void_p), ("col_ind", C.c_void_p), ("val", C.c_void_p),
+        ("cap_rows", C.c_int64), ("cap_nnz", C.c_int64),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("ms_total", C.c_double), ("ms_init", C.c_double), ("ms_insert", C.c_double),
+        ("ms_count", C.c_double), ("ms_scan", C.c_double), ("ms_write", C.c_double),
+        ("n_elems", C.c_int64), ("n_rows", C.c_int64), ("nnz", C.c_int64),
+        ("list_entries", C.c_int64), ("pool_chunks_used", C.c_int64), ("pool_chunks_cap", C.c_int64),
+        ("scratch_bytes", C.c_int64), ("chunk_slots", C.c_int32), ("retries", C.c_int32),
+        ("n_colors", C.c_int32), ("big_rows", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("n_streams", C.c_int32), ("hbm_budget_bytes", C.c_int64),
+        ("pattern_cache_entries", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [
+        ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
+        ("nnz", C.c_int64), ("cache_hit", C.c_int32), ("status", C.c_int32),
+    ]
+
+
+class GenSpec(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("jitter", C.c_double), ("seed", C.c_uint64), ("random_diagonal", C.c_int32),
+        ("permute_nodes", C.c_int32), ("split_fraction", C.c_double), ("E", C.c_double),
+        ("E_sigma", C.c_double), ("nu", C.c_double), ("thickness", C.c_double),
+    ]
+
+
+class HostMesh(C.Structure):
+    _fields_ = [("mesh", Mesh), ("structured_color", C.c_void_p), ("n_structured_colors", C.c_int32),
+                ("_pad", C.c_int32)]
+
+
+P = C.POINTER
+vp = C.c_void_p
+
+# name -> (restype, argtypes) for every function declared in include/alls.h.
+SIGNATURES = {
+    "alls_status_str": (C.c_char_p, [C.c_int]),
+    "alls_ctx_create": (C.c_int, [P(vp), C.c_int, vp]),
+    "alls_ctx_destroy": (None, [vp]),
+    "alls_ctx_last_error": (C.c_char_p, [vp]),
+    "alls_last_stats": (C.c_int, [vp, P(Stats)]),
+    "alls_device_count": (C.c_int, []),
+    "alls_assemble": (C.c_int, [vp, P(Mesh), P(Opts), P(Csr)]),
+    "alls_csr_download": (C.c_int, [vp, P(Csr), vp, vp, vp]),
+    "alls_csr_free": (None, [vp, P(Csr)]),
+    "alls_pattern_create": (C.c_int, [vp, P(Mesh), P(Opts), P(vp), P(Csr)]),
+    "alls_assemble_values": (C.c_int, [vp, vp, vp, C.c_int32, P(Csr)]),
+    "alls_pattern_nnz": (C.c_int64, [vp]),
+    "alls_pattern_free": (None, [vp, vp]),
+    "alls_color": (C.c_int, [vp, vp, P(vp)]),
+    "alls_coloring_from_host": (C.c_int, [vp, vp, vp, P(vp)]),
+    "alls_coloring_count": (C.c_int32, [vp]),
+    "alls_coloring_download": (C.c_int, [vp, vp, vp]),
+    "alls_coloring_free": (None, [vp, vp]),
+    "alls_assemble_colored": (C.c_int, [vp, vp, vp, vp, C.c_int32, P(Csr)]),
+    "alls_assemble_batch": (C.c_int, [vp, C.c_int32, P(Mesh), P(Opts), P(Csr)]),
+    "alls_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
+    "alls_server_submit": (C.c_int, [vp, P(Mesh), C.c_uint64, P(Opts), P(C.c_int64)]),
+    "alls_server_wait": (C.c_int, [vp, C.c_int64, P(Csr), P(RequestStats)]),
+    "alls_server_release": (None, [vp, P(Csr)]),
+    "alls_server_destroy": (None, [vp]),
+    "alls_gen_mesh": (C.c_int, [P(GenSpec), P(HostMesh)]),
+    "alls_host_mesh_free": (None, [P(HostMesh)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads liballs.so (built in-tree); raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class AllsError(RuntimeError):
+    def __init__(self, status, msg=""):
+        self.status = status
+        super().__init__(f"{lib().alls_status_str(status).decode()} ({status}): {msg}")
+
+
+def check(status, ctx=None):
+    if status != OK:
+        msg = ""
+        if ctx is not None:
+            m = lib().alls_ctx_last_error(ctx.handle)
+            msg = m.decode() if m else ""
+        raise AllsError(status, msg)
+
+
+# ----------------------------------------------------------- mesh handling --
+class MeshArrays:
+    """Owns numpy arrays of a mesh and exposes an alls_mesh view of them."""
+
+    def __init__(self, coords, conn, dim, nodes_per_elem, uniform_type=Q4, elem_type=None,
+                 E=1.0, E_per_elem=None, nu=0.3, thickness=1.0, structured_color=None,
+                 n_structured_colors=0):
+        self.coords = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1)
+        self.dim = int(dim)
+        self.npe = int(nodes_per_elem)
+        self.uniform_type = int(uniform_type)
+        self.elem_type = None if elem_type is None else np.ascontiguousarray(elem_type, dtype=np.uint8)
+        self.E = float(E)
+        self.E_per_elem = None if E_per_elem is None else np.ascontiguousarray(E_per_elem, dtype=np.float64)
+        self.nu = float(nu)
+        self.thickness = float(thickness)
+        self.structured_color = structured_color
+        self.n_structured_colors = n_structured_colors
+
+    @property
+    def n_nodes(self):
+        return self.coords.size // self.dim
+
+    @property
+    def n_elems(self):
+        return self.conn.size // self.npe
+
+    @property
+    def n_dofs(self):
+        return self.n_nodes * self.dim
+
+    def c_mesh(self):
+        m = Mesh()
+        m.dim, m.nodes_per_elem = self.dim, self.npe
+        m.n_nodes, m.n_elems = self.n_nodes, self.n_elems
+        m.coords = self.coords.ctypes.data
+        m.conn = self.conn.ctypes.data
+        m.elem_type = self.elem_type.ctypes.data if self.elem_type is not None else None
+        m.uniform_type, m.mem = self.uniform_type, MEM_HOST
+        m.E = self.E
+        m.E_per_elem = self.E_per_elem.ctypes.data if self.E_per_elem is not None else None
+        m.nu, m.thickness = self.nu, self.thickness
+        return m
+
+    def with_E(self, E_per_elem):
+        return MeshArrays(self.coords, self.conn, self.dim, self.npe, self.uniform_type, self.elem_type,
+                          self.E, E_per_elem, self.nu, self.thickness, self.structured_color,
+                          self.n_structured_colors)
+
+
+def generate(kind, nx, ny=None, nz=0, jitter=0.0, seed=0, random_diagonal=0, permute_nodes=0,
+             split_fraction=0.5, E=1.0, E_sigma=0.0, nu=0.3, thickness=1.0):
+    """Synthetic benchmark mesh from the library's generator (host only)."""
+    s = GenSpec()
+    s.kind, s.nx, s.ny, s.nz = kind, nx, ny if ny is not None else nx, nz
+    s.jitter, s.seed = jitter, seed
+    s.random_diagonal, s.permute_nodes, s.split_fraction = random_diagonal, permute_nodes, split_fraction
+    s.E, s.E_sigma, s.nu, s.thickness = E, E_sigma, nu, thickness
+    h = HostMesh()
+    check(lib().alls_gen_mesh(C.byref(s), C.byref(h)))
+    try:
+        m = h.mesh
+        dim, npe = m.dim, m.nodes_per_elem
+        nn, ne = m.n_nodes, m.n_elems
+
+        def arr(ptr, ctype, n):
+            if not ptr or n == 0:
+                return None
+            return np.ctypeslib.as_array(C.cast(ptr, P(ctype)), shape=(n,)).copy()
+
+        coords = arr(m.coords, C.c_double, nn * dim)
+        conn = arr(m.conn, C.c_int32, ne * npe)
+        et = arr(m.elem_type, C.c_uint8, ne)
+        Ee = arr(m.E_per_elem, C.c_double, ne)
+        col = arr(h.structured_color, C.c_int32, ne)
+        return MeshArrays(coords, conn, dim, npe, m.uniform_type, et, m.E, Ee, m.nu, m.thickness, col,
+                          h.n_structured_colors)
+    finally:
+        lib().alls_host_mesh_free(C.byref(h))
+
+
+# ---------------------------------------------------------------- context --
+class Context:
+    def __init__(self, device=0, stream=None):
+        h = vp()
+        check(lib().alls_ctx_create(C.byref(h), device, stream))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().alls_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        check(lib().alls_last_stats(self.handle, C.byref(s)), self)
+        return s.as_dict()
+
+    def assemble(self, mesh, precision=PREC_F64, chunk_slots=0, block_threads=0, row_ptr_i64=0,
+                 timing=1, out=None, c_mesh=None):
+        """Plain A-LLS assembly; returns a DeviceCsr (reused if `out` is given)."""
+        o = Opts()
+        o.precision, o.chunk_slots, o.block_threads = precision, chunk_slots, block_threads
+        o.row_ptr_i64, o.timing = row_ptr_i64, timing
+        out = out or DeviceCsr(self)
+        m = c_mesh if c_mesh is not None else mesh.c_mesh()
+        check(lib().alls_assemble(self.handle, C.byref(m), C.byref(o), C.byref(out.csr)), self)
+        return out
+
+
+class DeviceCsr:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.csr = Csr()
+
+    @property
+    def nnz(self):
+        return self.csr.nnz
+
+    @property
+    def n_rows(self):
+        return self.csr.n_rows
+
+    def download(self):
+        """(row_ptr int64, col_ind int32, val float64) as numpy arrays."""
+        c = self.csr
+        rp = np.empty(c.n_rows + 1, dtype=np.int64)
+        ci = np.empty(max(c.nnz, 1), dtype=np.int32)
+        v = np.empty(max(c.nnz, 1), dtype=np.float64)
+        check(lib().alls_csr_download(self.ctx.handle, C.byref(c), rp.ctypes.data, ci.ctypes.data,
+                                      v.ctypes.data), self.ctx)
+        return rp, ci[:c.nnz], v[:c.nnz]
+
+    def free(self):
+        if self.ctx is not None and self.ctx.handle:
+            lib().alls_csr_free(self.ctx.handle, C.byref(self.csr))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------- device-resident inputs --
+class _Hip:
+    """Minimal HIP runtime access (device buffers for device-resident meshes)."""
+
+    def __init__(self):
+        lib()  # loads libamdhip64 as a dependency of liballs
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.L = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        self.L.hipMalloc.argtypes = [P(vp), C.c_size_t]
+        self.L.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+        self.L.hipFree.argtypes = [vp]
+        self.L.hipDeviceSynchronize.argtypes = []
+        self.L.hipSetDevice.argtypes = [C.c_int]
+
+    def malloc(self, n):
+        p = vp()
+        r = self.L.hipMalloc(C.byref(p), max(int(n), 1))
+        if r != 0:
+            raise MemoryError(f"hipMalloc({n}) failed: {r}")
+        return p.value
+
+    def htod(self, dst, arr):
+        if arr.nbytes:
+            r = self.L.hipMemcpy(dst, arr.ctypes.data, arr.nbytes, 1)
+            if r != 0:
+                raise RuntimeError(f"hipMemcpy failed: {r}")
+
+    def free(self, p):
+        if p:
+            self.L.hipFree(p)
+
+    def sync(self):
+        return self.L.hipDeviceSynchronize()
+
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        _hip = _Hip()
+    return _hip
+
+
+class DeviceMesh:
+    """Copies a MeshArrays to device memory once; c_mesh() has mem=DEVICE."""
+
+    def __init__(self, mesh):
+        self.mesh = mesh
+        h = hip()
+        self.bufs = []
+
+        def up(a):
+            if a is None:
+                return None
+            p = h.malloc(a.nbytes)
+            h.htod(p, a)
+            self.bufs.append(p)
+            return p
+
+        self.coords = up(mesh.coords)
+        self.conn = up(mesh.conn)
+        self.elem_type = up(mesh.elem_type)
+        self.E_per_elem = up(mesh.E_per_elem)
+
+    def c_mesh(self, E_per_elem_ptr=None):
+        m = self.mesh.c_mesh()
+        m.coords, m.conn, m.elem_type = self.coords, self.conn, self.elem_type
+        m.E_per_elem = E_per_elem_ptr if E_per_elem_ptr is not None else self.E_per_elem
+        m.mem = MEM_DEVICE
+        return m
+
+    def free(self):
+        for p in self.bufs:
+            hip().free(p)
+        self.bufs = []
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

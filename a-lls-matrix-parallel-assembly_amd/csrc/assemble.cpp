@@ -356,8 +356,8 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
         st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
         st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
         st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
-        st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]) + ev_ms(ws.ev[5], ws.ev[6]) * 0.0;
-        st->ms_write = ev_ms(ws.ev[5], ws.ev[6]);
+        st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]);
+        st->ms_write = ev_ms(ws.ev[5], ws.ev[6]);   // scan apply + sort/merge/write
         st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);
         st->n_elems = dm.n_elems;
         st->n_rows = n_rows;

@@ -1,0 +1,202 @@
+"""Benchmark of the A-LLS assembly path on one MI355X.
+
+One step = one full A-LLS assembly (integration + list insertion + device
+consolidation into CSR) of the workload mesh, inputs already resident in
+HBM.  The default workload is the Q4 plane-stress 2000x2000 mesh (4.0M
+elements, 8.0M DOF, jitter +-0.2h, lognormal(0, 0.5) E per element).
+
+Prints one JSON line.  Extra keys: roofline of the dominant kernel (hipEvent
+times on the library's stream), the serial CPU baseline (the oracle on a
+bounded sample of the same workload, one thread), and secondary
+measurements (Hex20 FP64 vs FP32-MFMA, GPU coloring, serving) under "extra".
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "a-lls-matrix-parallel-assembly_amd"))
+
+METRIC = "elements/s & %HBM roofline vs GPU coloring; FP32-MFMA speedup; served p99 ms"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def q4_workload(n, seed=0xA11500C2):
+    import alls
+    return alls.generate(alls.GEN_Q4, n, n, jitter=0.2, seed=seed, E_sigma=0.5, nu=0.3, thickness=1.0)
+
+
+def hex20_workload(n, seed=0xA11500C4):
+    import alls
+    return alls.generate(alls.GEN_HEX20, n, n, n, jitter=0.15, seed=seed, E=1.0, nu=0.3)
+
+
+def time_steps(ctx, dmesh, steps, warmup, **opts):
+    """W untimed + K timed assemblies; returns (seconds, mean stats, out)."""
+    import alls
+    cm = dmesh.c_mesh()
+    out = None
+    for _ in range(warmup):
+        out = ctx.assemble(dmesh.mesh, out=out, c_mesh=cm, **opts)
+    alls.hip().sync()
+    acc = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = ctx.assemble(dmesh.mesh, out=out, c_mesh=cm, **opts)
+        st = ctx.stats()
+        for k in ("ms_total", "ms_init", "ms_insert", "ms_count", "ms_scan", "ms_write"):
+            acc[k] = acc.get(k, 0.0) + st[k]
+    alls.hip().sync()
+    dt = time.perf_counter() - t0
+    st = ctx.stats()
+    for k in acc:
+        st[k] = acc[k] / steps
+    return dt, st, out
+
+
+def path_bytes(mesh, nnz):
+    """Compulsory bytes of the whole path: inputs + output CSR."""
+    inputs = mesh.coords.nbytes + mesh.conn.nbytes + (mesh.E_per_elem.nbytes if mesh.E_per_elem is not None else 0)
+    return inputs + 4 * (mesh.n_dofs + 1) + 12 * nnz
+
+
+def kernel_bytes(mesh, st, entry_bytes):
+    """Algorithmic bytes per launch of each pipeline stage."""
+    inputs = mesh.coords.nbytes + mesh.conn.nbytes + (mesh.E_per_elem.nbytes if mesh.E_per_elem is not None else 0)
+    entries = st["list_entries"]
+    nnz = st["nnz"]
+    n_nodes = mesh.n_nodes
+    return {
+        # reads the mesh, appends every list entry (key 8 B + values)
+        "insert": inputs + entries * entry_bytes + 4 * n_nodes,
+        # reads each row's keys, writes one count per row
+        "count": entries * 8 + 8 * n_nodes,
+        # reads keys and values, writes col_ind (4 B) + val (8 B) per nonzero and row_ptr
+        "write": entries * entry_bytes + 12 * nnz + 8 * (mesh.n_dofs + 1),
+    }
+
+
+def roofline(mesh, st, entry_bytes):
+    kb = kernel_bytes(mesh, st, entry_bytes)
+    times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
+    dom = max(times, key=times.get)
+    achieved = kb[dom] / (times[dom] * 1e-3) / 1e9
+    return dom, {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    }, kb, times
+
+
+def cpu_baseline(n):
+    """Oracle (serial restatement of Acc:329-380 + sorted pattern) on one core."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    m = q4_workload(n)
+    t0 = time.perf_counter()
+    rp, ci = oracle.pattern(m)
+    t1 = time.perf_counter()
+    oracle.assemble(m, rp, ci, with_abs=False)
+    t2 = time.perf_counter()
+    return {"value": round(m.n_elems / (t2 - t0), 1), "unit": "elements/s", "cores": 1, "kind": "port",
+            "sample": f"Q4 {n}x{n} ({m.n_elems} elements) of the same generator; pattern {t1 - t0:.2f} s + "
+                      f"serial fill {t2 - t1:.2f} s",
+            "seconds": round(t2 - t0, 2)}
+
+
+def hex20_compare(ctx, n, steps):
+    """FP64 vs FP32-MFMA Hex20 assembly of an n^3 brick (one GPU)."""
+    import alls
+    m = hex20_workload(n)
+    dm = alls.DeviceMesh(m)
+    res = {}
+    v = {}
+    for name, prec in (("f64", alls.PREC_F64), ("f32_mfma", alls.PREC_F32_MFMA)):
+        dt, st, out = time_steps(ctx, dm, steps, 1, precision=prec)
+        res[name] = {"ms_per_step": round(dt / steps * 1e3, 3), "elements_per_s": round(m.n_elems * steps / dt, 1),
+                     "ms_insert": round(st["ms_insert"], 3), "ms_count": round(st["ms_count"], 3),
+                     "ms_write": round(st["ms_write"], 3), "nnz": st["nnz"]}
+        v[name] = out.download()[2] if n <= 40 else None
+        out.free()
+    dm.free()
+    res["speedup_f32_vs_f64"] = round(res["f64"]["ms_per_step"] / res["f32_mfma"]["ms_per_step"], 3)
+    if v["f64"] is not None:
+        res["frobenius_f32_vs_f64"] = float(np.linalg.norm(v["f32_mfma"] - v["f64"]) / np.linalg.norm(v["f64"]))
+    res["workload"] = f"Hex20 {n}^3 brick, jitter +-0.15h, E=1, nu=0.3, 3x3x3 Gauss"
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=2000, help="Q4 grid size (workload n x n)")
+    ap.add_argument("--cpu-n", type=int, default=1000, help="CPU baseline sample grid size")
+    ap.add_argument("--hex-n", type=int, default=80)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import alls
+    log(f"generating Q4 {a.n}x{a.n}")
+    mesh = q4_workload(a.n)
+    ctx = alls.Context(0)
+    dm = alls.DeviceMesh(mesh)
+    log("timing A-LLS steps")
+    dt, st, out = time_steps(ctx, dm, a.steps, a.warmup)
+    ms = dt / a.steps * 1e3
+    value = mesh.n_elems * a.steps / dt
+    dom, roof, kb, times = roofline(mesh, st, 8 + 4 * 8)
+    pb = path_bytes(mesh, st["nnz"])
+    log(f"{value:.3e} elements/s, {ms:.3f} ms/step, stages {times}")
+    out.free()
+    res = {
+        "metric": METRIC, "value": round(value, 1), "unit": "elements/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"A-LLS assembly, Q4 plane stress {a.n}x{a.n} elements ({mesh.n_elems} elements, "
+                               f"{mesh.n_dofs} DOF, nnz {st['nnz']}), jitter +-0.2h, E~lognormal(0,0.5), "
+                               f"nu=0.3, t=1, 2x2 Gauss, FP64, inputs resident in HBM",
+                   "n_elems": mesh.n_elems, "n_dofs": mesh.n_dofs, "nnz": st["nnz"]},
+        "roofline": dict(roof, kernel=dom,
+                         algorithmic_bytes_per_launch=kb[dom],
+                         note="algorithmic bytes: mesh reads + list entries (insert); keys+values read and "
+                              "CSR written (write)"),
+        "stages_ms": {k: round(v, 4) for k, v in times.items()},
+        "path": {"compulsory_bytes": pb, "achieved_GBs": round(pb / (ms * 1e-3) / 1e9, 1),
+                 "frac_of_hbm_peak": round(pb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    dm.free()
+    if not a.no_cpu:
+        log(f"cpu baseline on Q4 {a.cpu_n}x{a.cpu_n}")
+        res["cpu_baseline"] = cpu_baseline(a.cpu_n)
+        res["cpu_baseline"]["gpu_speedup"] = round(value / res["cpu_baseline"]["value"], 1)
+    extra = {}
+    if not a.no_extra:
+        try:
+            log(f"Hex20 {a.hex_n}^3 FP64 vs FP32-MFMA")
+            extra["hex20"] = hex20_compare(ctx, a.hex_n, 3)
+        except Exception as e:  # report, do not hide the main line
+            extra["hex20"] = {"error": str(e)}
+    if extra:
+        res["extra"] = extra
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()
