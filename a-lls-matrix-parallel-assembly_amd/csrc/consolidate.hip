@@ -26,8 +26,12 @@ __device__ __forceinline__ uint64_t pack_key(int2 kv) {
 __device__ __forceinline__ int32_t key_col(uint64_t k) { return (int32_t)(k >> 32); }
 
 __device__ __forceinline__ int64_t entry_index(const ListPool& p, int64_t a, const int32_t* h, int32_t s) {
+    // Callers clamp s below ALLS_DIR * C.  A chunk that was never published
+    // (only after a directory overflow, which the host then retries) maps to
+    // the home chunk so that every access stays in bounds.
     const int32_t k = s / p.C;
-    const int64_t ch = k == 0 ? a : p.n_nodes + (int64_t)(h[k] - 1);
+    const int32_t d = k == 0 ? 0 : h[k];
+    const int64_t ch = d > 0 ? p.n_nodes + (int64_t)(d - 1) : a;
     return ch * p.C + (s % p.C);
 }
 
@@ -41,7 +45,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     const int lane = (int)(gid % G);
     if (a >= p.n_nodes) return;
     const int32_t* h = p.hdr + a * ALLS_DIR;
-    const int32_t n = h[0];
+    const int32_t n = min(h[0], ALLS_DIR * p.C);
     if (n > G) {
         if (!WRITE && lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
         return;
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list
     for (int64_t r = blockIdx.x; r < count; r += gridDim.x) {
         const int64_t a = list ? (int64_t)list[r] : r;
         const int32_t* h = p.hdr + a * ALLS_DIR;
-        const int32_t n = h[0];
+        const int32_t n = min(h[0], ALLS_DIR * p.C);
         if (n > cap) {
             if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], n);
             continue;
