@@ -66,6 +66,18 @@ __device__ __forceinline__ f32x4 mfma(float a, float b, f32x4 c) {
 template <typename T> struct Acc;
 template <> struct Acc<double> { typedef f64x4 type; };
 template <> struct Acc<float> { typedef f32x4 type; };
+// Row of the 16x16 output held in register q of lane (li + 16*lk); the
+// column is li.  Measured on gfx950: the FP64 16x16x4 instruction
+// interleaves rows across lane groups (row 4q + lk), the FP32 one keeps four
+// consecutive rows per lane group (row 4lk + q).  A and B take A[i][k] and
+// B[k][j] from lane i + 16k / j + 16k in both.
+template <typename T> struct OutRow;
+template <> struct OutRow<double> {
+    static __device__ __forceinline__ int row(int lk, int q) { return 4 * q + lk; }
+};
+template <> struct OutRow<float> {
+    static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
+};
 
 template <typename T>
 __global__ __launch_bounds__(64) void k_insert_hex20(DevMesh m, ListPool p) {
@@ -179,7 +191,7 @@ __global__ __launch_bounds__(64) void k_insert_hex20(DevMesh m, ListPool p) {
             for (int tc = tr; tc < 4; ++tc) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int r = 16 * tr + 4 * lk + q, c = 16 * tc + li;
+                    const int r = 16 * tr + OutRow<T>::row(lk, q), c = 16 * tc + li;
                     if (r < 60 && c < 60) {
                         M[r * MS + c] = acc[t][q];
                         if (tr != tc) M[c * MS + r] = acc[t][q];
