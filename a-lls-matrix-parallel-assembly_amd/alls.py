@@ -39,7 +39,7 @@ class Opts(C.Structure):
 
 class Csr(C.Structure):
     _fields_ = [
-        ("n_rows", C.c_int64), ("nnz", C.c_int64), ("row_ptr_is_i64", C.c_int32), ("_pad", C.c_int32),
+        ("n_rows", C.c_int64), ("nnz", C.c_int64), ("row_ptr_is_i64", C.c_int32), ("pattern_id", C.c_int32),
         ("row_ptr", C.c_void_p), ("col_ind", C.c_void_p), ("val", C.c_void_p),
         ("cap_rows", C.c_int64), ("cap_nnz", C.c_int64),
     ]
@@ -275,6 +275,96 @@ class Context:
         m = c_mesh if c_mesh is not None else mesh.c_mesh()
         check(lib().alls_assemble(self.handle, C.byref(m), C.byref(o), C.byref(out.csr)), self)
         return out
+
+
+class Pattern:
+    """Cached sparsity pattern + scatter map (alls_pattern_create)."""
+
+    def __init__(self, ctx, mesh, precision=PREC_F64, chunk_slots=0, out=None, c_mesh=None):
+        self.ctx = ctx
+        o = Opts()
+        o.precision, o.chunk_slots, o.timing = precision, chunk_slots, 1
+        self.out = out or DeviceCsr(ctx)
+        h = vp()
+        m = c_mesh if c_mesh is not None else mesh.c_mesh()
+        check(lib().alls_pattern_create(ctx.handle, C.byref(m), C.byref(o), C.byref(h), C.byref(self.out.csr)), ctx)
+        self.handle = h
+        self.mesh = mesh
+
+    @property
+    def nnz(self):
+        return lib().alls_pattern_nnz(self.handle)
+
+    def values(self, E_per_elem=None, out=None, device_ptr=None):
+        """Value-only re-assembly with a new E field (numpy array or device pointer)."""
+        out = out or DeviceCsr(self.ctx)
+        if device_ptr is not None:
+            check(lib().alls_assemble_values(self.ctx.handle, self.handle, device_ptr, MEM_DEVICE,
+                                             C.byref(out.csr)), self.ctx)
+        else:
+            E = None if E_per_elem is None else np.ascontiguousarray(E_per_elem, np.float64)
+            check(lib().alls_assemble_values(self.ctx.handle, self.handle, None if E is None else E.ctypes.data,
+                                             MEM_HOST, C.byref(out.csr)), self.ctx)
+        return out
+
+    def color_gpu(self):
+        h = vp()
+        check(lib().alls_color(self.ctx.handle, self.handle, C.byref(h)), self.ctx)
+        return Coloring(self.ctx, h)
+
+    def color_from_host(self, colors):
+        colors = np.ascontiguousarray(colors, np.int32)
+        h = vp()
+        check(lib().alls_coloring_from_host(self.ctx.handle, self.handle, colors.ctypes.data, C.byref(h)), self.ctx)
+        return Coloring(self.ctx, h)
+
+    def colored(self, coloring, E_per_elem=None, out=None, device_ptr=None):
+        out = out or DeviceCsr(self.ctx)
+        if device_ptr is not None:
+            check(lib().alls_assemble_colored(self.ctx.handle, self.handle, coloring.handle, device_ptr, MEM_DEVICE,
+                                              C.byref(out.csr)), self.ctx)
+        else:
+            E = None if E_per_elem is None else np.ascontiguousarray(E_per_elem, np.float64)
+            check(lib().alls_assemble_colored(self.ctx.handle, self.handle, coloring.handle,
+                                              None if E is None else E.ctypes.data, MEM_HOST, C.byref(out.csr)),
+                  self.ctx)
+        return out
+
+    def free(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            lib().alls_pattern_free(self.ctx.handle, self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Coloring:
+    def __init__(self, ctx, handle):
+        self.ctx, self.handle = ctx, handle
+
+    @property
+    def n_colors(self):
+        return lib().alls_coloring_count(self.handle)
+
+    def download(self, n_elems):
+        c = np.empty(n_elems, np.int32)
+        check(lib().alls_coloring_download(self.ctx.handle, self.handle, c.ctypes.data), self.ctx)
+        return c
+
+    def free(self):
+        if getattr(self, "handle", None) and self.ctx.handle:
+            lib().alls_coloring_free(self.ctx.handle, self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class DeviceCsr:

@@ -59,8 +59,27 @@ struct ListPool {
     int32_t B;                 // values per entry
 };
 
+// Element -> CSR-slot scatter of a cached pattern.  map[e*npe2 + i*npe + j]
+// is the rank of column node conn[e][j] among the sorted unique column nodes
+// of node row conn[e][i]; DOF entry (dr, dc) of that block sits at
+// row_ptr[a*dpn] + rank*dpn + dr*rowlen + dc with rowlen = row_ptr[a*dpn+1] -
+// row_ptr[a*dpn].
+struct Scatter {
+    const int32_t* map;
+    const void* row_ptr;
+    double* val;
+    const int32_t* elist;      // elements to process, or null => 0..count-1
+    int64_t count;
+    int32_t i64;               // row_ptr is int64
+    int32_t npe;               // map stride per node (4 in 2D, 20 for Hex20)
+};
+
 #ifdef __HIPCC__
 #define ALLS_DEV __device__ __forceinline__
+
+ALLS_DEV int64_t rp_at(const void* rp, int i64, int64_t i) {
+    return i64 ? static_cast<const int64_t*>(rp)[i] : (int64_t) static_cast<const int32_t*>(rp)[i];
+}
 
 ALLS_DEV int32_t ld_agent(const int32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -130,11 +149,6 @@ ALLS_DEV int64_t list_chunk(const ListPool& p, int32_t a, int32_t s) {
     return p.n_nodes + (int64_t)(v - 1);
 }
 
-// Chunk of slot s when every chunk is already published (after insertion).
-ALLS_DEV int64_t list_chunk_done(const ListPool& p, const int32_t* dir, int32_t s) {
-    int32_t k = s / p.C;
-    return k == 0 ? -1 : p.n_nodes + (int64_t)(dir[k] - 1);
-}
 #endif  // __HIPCC__
 
 // ----------------------------------------------------------- host side ---
@@ -148,6 +162,23 @@ namespace alls {
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, hipStream_t s);
 void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, hipStream_t s);
+// Pattern cache: build the scatter map; value-only and coloured scatter.
+void launch_build_map(const DevMesh& m, int kind, const void* row_ptr, int i64, const int32_t* col_ind,
+                      int32_t* map, int32_t* misc, hipStream_t s);
+// atomic = 1: FP64 atomics (all elements at once); 0: plain "+=" (one colour).
+void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic, int32_t* misc, hipStream_t s);
+void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int atomic, int f32, int32_t* misc, hipStream_t s);
+void launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
+// Colouring: node -> element incidence, JPL rounds, conflict check, buckets.
+void launch_incidence(const DevMesh& m, int32_t* cnt, int64_t* off, int32_t* cursor, int32_t* list, int64_t* tiles,
+                      hipStream_t s);
+void launch_jpl_round(const DevMesh& m, const int64_t* off, const int32_t* list, const int32_t* cin, int32_t* cout,
+                      int32_t* misc, hipStream_t s);
+void launch_conflicts(const DevMesh& m, const int64_t* off, const int32_t* list, const int32_t* col, int32_t* misc,
+                      hipStream_t s);
+void launch_color_count(const int32_t* col, int64_t n, int32_t* cnt, hipStream_t s);
+void launch_color_bucket(const int32_t* col, int64_t n, const int64_t* coff, int32_t* cursor, int32_t* elist,
+                         hipStream_t s);
 // Consolidation.  small_group = lanes per row of the register kernel (0 =>
 // every row goes to the wave-per-row LDS kernel).
 void launch_count(const ListPool& p, int small_group, int f32vals, int32_t* u, int32_t* big_list,

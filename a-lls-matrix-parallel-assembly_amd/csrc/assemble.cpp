@@ -210,6 +210,7 @@ alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
             return ALLS_E_OOM;
         }
         out->cap_rows = n_rows + 1;
+        out->pattern_id = 0;
     }
     if (out->cap_nnz < nnz || !out->col_ind || !out->val) {
         if (out->col_ind) (void)hipFree(out->col_ind);
@@ -224,6 +225,7 @@ alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
             return ALLS_E_OOM;
         }
         out->cap_nnz = n;
+        out->pattern_id = 0;
     }
     out->n_rows = n_rows;
     out->nnz = nnz;
@@ -351,6 +353,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     launch_write(lp, G, mi.f32 ? 1 : 0, dpn, ws.big.as<int32_t>(), out->row_ptr, i64, out->col_ind, out->val, s);
     HIP_TRY(hipGetLastError(), "launch write");
     HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+    out->pattern_id = 0;
     HIP_TRY(hipStreamSynchronize(s), "sync after write");
     if (st) {
         st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);

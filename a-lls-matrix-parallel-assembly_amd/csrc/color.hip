@@ -1,0 +1,170 @@
+// color.hip -- element colouring on the GPU and colour bucketing (gfx950).
+//
+// Replaces the host greedy colourers (Acc:648-747 degree-ordered,
+// T3:484-523 and Hex:394-417 natural order), which are O(N^2) in memory
+// (a std::vector<bool> of N per element).  Elements are adjacent when they
+// share a node.  Jones-Plassmann-Luby rounds: an uncoloured element whose
+// (hash priority, id) beats every uncoloured neighbour takes the smallest
+// colour no coloured neighbour has.  Each round reads the previous round's
+// colours and writes a new array, so the result does not depend on thread
+// timing; two adjacent elements never colour in the same round.
+#include "alls_internal.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t prio(uint32_t e) {
+    uint32_t h = e * 0x9E3779B1u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+__device__ __forceinline__ int elem_nodes(const DevMesh& m, int64_t e) {
+    const int kind = m.types ? (int)m.types[e] : m.uniform_type;
+    return kind == 0 ? 4 : kind == 1 ? 3 : 20;
+}
+
+__global__ void k_inc_count(DevMesh m, int32_t* cnt) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_elems) return;
+    const int nn = elem_nodes(m, e);
+    for (int i = 0; i < nn; ++i) atomicAdd(&cnt[m.conn[e * m.npe + i]], 1);
+}
+
+__global__ void k_inc_fill(DevMesh m, const int64_t* off, int32_t* cursor, int32_t* list) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_elems) return;
+    const int nn = elem_nodes(m, e);
+    for (int i = 0; i < nn; ++i) {
+        const int32_t a = m.conn[e * m.npe + i];
+        list[off[a] + atomicAdd(&cursor[a], 1)] = (int32_t)e;
+    }
+}
+
+// One JPL round.  misc[0] counts elements still uncoloured after the round,
+// misc[1] is the largest colour used.
+__global__ void k_jpl_round(DevMesh m, const int64_t* off, const int32_t* list, const int32_t* cin, int32_t* cout,
+                            int32_t* misc) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_elems) return;
+    const int32_t ce = cin[e];
+    if (ce >= 0) {
+        cout[e] = ce;
+        return;
+    }
+    const uint32_t pe = prio((uint32_t)e);
+    const int nn = elem_nodes(m, e);
+    bool best = true;
+    for (int i = 0; i < nn && best; ++i) {
+        const int32_t a = m.conn[e * m.npe + i];
+        for (int64_t k = off[a]; k < off[a + 1]; ++k) {
+            const int32_t f = list[k];
+            if (f == e || cin[f] >= 0) continue;
+            const uint32_t pf = prio((uint32_t)f);
+            if (pf > pe || (pf == pe && f > e)) {
+                best = false;
+                break;
+            }
+        }
+    }
+    if (!best) {
+        cout[e] = -1;
+        atomicAdd(&misc[0], 1);
+        return;
+    }
+    // Smallest colour absent from the coloured neighbours, 64 colours at a time.
+    for (int32_t base = 0;; base += 64) {
+        uint64_t used = 0;
+        for (int i = 0; i < nn; ++i) {
+            const int32_t a = m.conn[e * m.npe + i];
+            for (int64_t k = off[a]; k < off[a + 1]; ++k) {
+                const int32_t c = cin[list[k]] - base;
+                if (c >= 0 && c < 64) used |= 1ull << c;
+            }
+        }
+        if (~used) {
+            const int32_t c = base + __ffsll((long long)~used) - 1;
+            cout[e] = c;
+            atomicMax(&misc[1], c);
+            return;
+        }
+    }
+}
+
+// Counts colour conflicts (adjacent elements with equal colour) and negative colours.
+__global__ void k_conflicts(DevMesh m, const int64_t* off, const int32_t* list, const int32_t* col, int32_t* misc) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.n_elems) return;
+    const int32_t ce = col[e];
+    if (ce < 0) {
+        atomicAdd(&misc[0], 1);
+        return;
+    }
+    atomicMax(&misc[1], ce);
+    const int nn = elem_nodes(m, e);
+    for (int i = 0; i < nn; ++i) {
+        const int32_t a = m.conn[e * m.npe + i];
+        for (int64_t k = off[a]; k < off[a + 1]; ++k) {
+            const int32_t f = list[k];
+            if (f != e && col[f] == ce) {
+                atomicAdd(&misc[0], 1);
+                return;
+            }
+        }
+    }
+}
+
+__global__ void k_color_count(const int32_t* col, int64_t n, int32_t* cnt) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) atomicAdd(&cnt[col[e]], 1);
+}
+
+// Buckets elements by colour.  The order inside a colour is arbitrary: an
+// entry receives at most one contribution per colour, so coloured sums do
+// not depend on it.
+__global__ void k_color_bucket(const int32_t* col, int64_t n, const int64_t* coff, int32_t* cursor, int32_t* elist) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int32_t c = col[e];
+    elist[coff[c] + atomicAdd(&cursor[c], 1)] = (int32_t)e;
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+
+}  // namespace
+
+namespace alls {
+
+void launch_incidence(const DevMesh& m, int32_t* cnt, int64_t* off, int32_t* cursor, int32_t* list, int64_t* tiles,
+                      hipStream_t s) {
+    launch_fill_i32(cnt, m.n_nodes, 0, s);
+    launch_fill_i32(cursor, m.n_nodes, 0, s);
+    if (m.n_elems > 0) hipLaunchKernelGGL(k_inc_count, dim3(grid_for(m.n_elems)), dim3(256), 0, s, m, cnt);
+    launch_scan_total(cnt, m.n_nodes, tiles, scan_tile_count(m.n_nodes), s);
+    launch_scan_apply(cnt, m.n_nodes, tiles, 1, off, 1, s);
+    if (m.n_elems > 0) hipLaunchKernelGGL(k_inc_fill, dim3(grid_for(m.n_elems)), dim3(256), 0, s, m, off, cursor, list);
+}
+
+void launch_jpl_round(const DevMesh& m, const int64_t* off, const int32_t* list, const int32_t* cin, int32_t* cout,
+                      int32_t* misc, hipStream_t s) {
+    hipLaunchKernelGGL(k_jpl_round, dim3(grid_for(m.n_elems)), dim3(256), 0, s, m, off, list, cin, cout, misc);
+}
+
+void launch_conflicts(const DevMesh& m, const int64_t* off, const int32_t* list, const int32_t* col, int32_t* misc,
+                      hipStream_t s) {
+    if (m.n_elems > 0) hipLaunchKernelGGL(k_conflicts, dim3(grid_for(m.n_elems)), dim3(256), 0, s, m, off, list, col, misc);
+}
+
+void launch_color_count(const int32_t* col, int64_t n, int32_t* cnt, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_color_count, dim3(grid_for(n)), dim3(256), 0, s, col, n, cnt);
+}
+
+void launch_color_bucket(const int32_t* col, int64_t n, const int64_t* coff, int32_t* cursor, int32_t* elist,
+                         hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_color_bucket, dim3(grid_for(n)), dim3(256), 0, s, col, n, coff, cursor, elist);
+}
+
+}  // namespace alls

@@ -10,10 +10,25 @@
 
 namespace alls {
 
-// Grow-only device buffer.
+// Grow-only device buffer (owns its allocation; movable, not copyable).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
     hipError_t ensure(size_t need);
     void release();
     template <class T> T* as() const { return static_cast<T*>(p); }
@@ -51,6 +66,13 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
 alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64);
 void free_csr(alls_csr* out);
 alls_status hip_fail(hipError_t e, const char* what, std::string* err);
+// Pattern cache internals shared with the server (pattern.cpp).
+alls_status pattern_build(Workspace& ws, const alls_mesh* mesh, const alls_opts& o, hipStream_t s, int device,
+                          alls_pattern** out_pat, alls_csr* out, alls_stats* st, std::string* err);
+alls_status pattern_values(Workspace& ws, alls_pattern* p, const double* E_dev, alls_csr* out, hipStream_t s,
+                           alls_stats* st, std::string* err);
+void pattern_destroy(alls_pattern* p);
+int64_t pattern_bytes(const alls_pattern* p);
 
 }  // namespace alls
 

@@ -1,28 +1,29 @@
-// insert_2d.hip -- Q4 / T3 / mixed plane-stress element integration fused
-// with A-LLS list insertion (gfx950).
+// insert_2d.hip -- Q4 / T3 / mixed plane-stress kernels (gfx950):
+//   k_insert_2d  : element integration fused with A-LLS list insertion;
+//   k_map        : element -> CSR-slot map of a cached pattern;
+//   k_scatter_2d : value-only (FP64 atomics) or coloured (plain "+=")
+//                  scatter through that map.
 //
-// Replaces assembleGlobalStiffnessKernelALLS (Acc:394-468, T3:227-290,
-// Mix:248-312).  The reference walks each row's linked list per scalar
-// contribution and merges equal columns with FP64 atomics.  Here one thread
-// integrates one element, keeps the Gauss-point moments
-//   Sxx_ab = sum w bx_a bx_b,  Syy_ab = sum w by_a by_b,  Sxy_ab = sum w bx_a by_b
-// in registers, and appends, for each of its row nodes a, one entry per
-// column node b carrying the 2x2 block K_ab: one reservation atomic per
-// (element, row node) instead of a list walk and an atomic per scalar.
-// Equal columns are merged later by the per-row sort in consolidation.
-#include "alls_internal.h"
+// k_insert_2d replaces assembleGlobalStiffnessKernelALLS (Acc:394-468,
+// T3:227-290, Mix:248-312).  The reference walks each row's linked list per
+// scalar contribution and merges equal columns with FP64 atomics.  Here one
+// thread integrates one element (Elem2D) and, for each of its row nodes a,
+// appends one entry per column node b carrying the 2x2 block K_ab: one
+// reservation atomic per (element, row node) instead of a list walk and an
+// atomic per scalar.  Equal columns are merged by the per-row sort in
+// consolidation.  k_scatter_2d replaces the coloured kernels
+// (Acc:504-572, T3:316-363, Mix:341-390) and the value-only re-assembly.
+#include "elem2d.h"
+
+using alls_dev::Elem2D;
 
 namespace {
 
-// Gauss abscissa of the reference's 2-point rule, Acc:47 (same literal).
-constexpr double GP2 = 0.577350269189626;
-
-__device__ __forceinline__ void store_entry(const ListPool& p, int64_t idx, int32_t col, int32_t eid,
-                                            double k00, double k01, double k10, double k11) {
+__device__ __forceinline__ void store_entry(const ListPool& p, int64_t idx, int32_t col, int32_t eid, const double* k) {
     p.keys[idx] = make_int2(col, eid);
     double2* v = reinterpret_cast<double2*>(static_cast<double*>(p.vals) + idx * 4);
-    v[0] = make_double2(k00, k01);
-    v[1] = make_double2(k10, k11);
+    v[0] = make_double2(k[0], k[1]);
+    v[1] = make_double2(k[2], k[3]);
 }
 
 // MODE 0: all Q4, 1: all T3, 2: mixed (per-element type array).
@@ -30,114 +31,19 @@ template <int MODE>
 __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
     const int64_t e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (e >= m.n_elems) return;
-    const int kind = MODE == 2 ? (int)m.types[e] : MODE;
-    const int nn = kind == 0 ? 4 : 3;
-    if (kind > 1) {
-        st_agent(&p.misc[MISC_ERR + ERR_INVALID_MESH], 1);
-        return;
-    }
-
-    int32_t nd[4];
-    const int32_t* c = m.conn + e * m.npe;
-    if (MODE != 1 && m.npe == 4) {
-        int4 q = *reinterpret_cast<const int4*>(c);
-        nd[0] = q.x; nd[1] = q.y; nd[2] = q.z; nd[3] = q.w;
-    } else {
-        nd[0] = c[0]; nd[1] = c[1]; nd[2] = c[2]; nd[3] = nn == 4 ? c[3] : 0;
-    }
-    bool ok = true;
-    for (int i = 0; i < nn; ++i) ok &= (nd[i] >= 0) & (nd[i] < m.n_nodes);
-    if (!ok) {
-        st_agent(&p.misc[MISC_ERR + ERR_INVALID_MESH], 1);
-        return;
-    }
-    double x[4], y[4];
-    for (int i = 0; i < nn; ++i) {
-        double2 q = *reinterpret_cast<const double2*>(m.coords + 2 * (int64_t)nd[i]);
-        x[i] = q.x;
-        y[i] = q.y;
-    }
-    const double E = m.E_elem ? m.E_elem[e] : m.E;
-    const double nu = m.nu;
-    const double f = E / (1.0 - nu * nu);        // D = f [[1,nu,0],[nu,1,0],[0,0,(1-nu)/2]]
-    const double cs = (1.0 - nu) / 2.0;
-
-    // Moments over the Gauss points (upper triangles of Sxx, Syy; full Sxy).
-    double Sxx[4][4], Syy[4][4], Sxy[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) Sxx[a][b] = Syy[a][b] = Sxy[a][b] = 0.0;
-
-    if (kind == 0) {
-        // 2x2 Gauss, weights 1 (Acc:47-48); detJ clamped at 1e-10 (Acc:143).
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const double xi = (g & 2) ? GP2 : -GP2;
-            const double eta = (g & 1) ? GP2 : -GP2;
-            const double dxi[4] = {-0.25 * (1.0 - eta), 0.25 * (1.0 - eta), 0.25 * (1.0 + eta), -0.25 * (1.0 + eta)};
-            const double deta[4] = {-0.25 * (1.0 - xi), -0.25 * (1.0 + xi), 0.25 * (1.0 + xi), 0.25 * (1.0 - xi)};
-            double J0 = 0, J1 = 0, J2 = 0, J3 = 0;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                J0 += dxi[i] * x[i];
-                J1 += dxi[i] * y[i];
-                J2 += deta[i] * x[i];
-                J3 += deta[i] * y[i];
-            }
-            double det = J0 * J3 - J1 * J2;
-            if (fabs(det) < 1e-10) det = 1e-10;
-            const double i0 = J3 / det, i1 = -J1 / det, i2 = -J2 / det, i3 = J0 / det;
-            double bx[4], by[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                bx[i] = i0 * dxi[i] + i1 * deta[i];
-                by[i] = i2 * dxi[i] + i3 * deta[i];
-            }
-            const double w = det * m.thickness;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const double wx = w * bx[a], wy = w * by[a];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    if (b >= a) {
-                        Sxx[a][b] += wx * bx[b];
-                        Syy[a][b] += wy * by[b];
-                    }
-                    Sxy[a][b] += wx * by[b];
-                }
-            }
-        }
-    } else {
-        // Constant-strain triangle, one point, weight 0.5*detJ*t (T3:177-205).
-        const double J0 = x[1] - x[0], J1 = y[1] - y[0], J2 = x[2] - x[0], J3 = y[2] - y[0];
-        const double det = J0 * J3 - J1 * J2;
-        const double i0 = J3 / det, i1 = -J1 / det, i2 = -J2 / det, i3 = J0 / det;
-        const double bx[3] = {-i0 - i1, i0, i1};
-        const double by[3] = {-i2 - i3, i2, i3};
-        const double w = 0.5 * det * m.thickness;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                if (b >= a) {
-                    Sxx[a][b] = w * bx[a] * bx[b];
-                    Syy[a][b] = w * by[a] * by[b];
-                }
-                Sxy[a][b] = w * bx[a] * by[b];
-            }
-    }
-
+    Elem2D el;
+    if (!el.load<MODE>(m, e, p.misc)) return;
+    const int nn = el.nn;
     // Phase 1: reserve nn slots in each of the nn node rows.
     int32_t pos[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (i < nn) pos[i] = list_reserve(p, nd[i], nn);
+        if (i < nn) pos[i] = list_reserve(p, el.nd[i], nn);
     // Phase 2: publish the overflow chunks these reservations own.  Every lane
     // of the wave finishes this before any lane waits on a chunk in phase 3.
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (i < nn) list_own_chunks(p, nd[i], pos[i], nn);
+        if (i < nn) list_own_chunks(p, el.nd[i], pos[i], nn);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     // Phase 3: write the entries.
 #pragma unroll
@@ -147,15 +53,80 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
         for (int j = 0; j < 4; ++j) {
             if (j >= nn) continue;
             const int32_t s = pos[i] + j;
-            const int64_t ch = list_chunk(p, nd[i], s);
+            const int64_t ch = list_chunk(p, el.nd[i], s);
             if (ch < 0) continue;
-            const int lo = i < j ? i : j, hi = i < j ? j : i;
-            const double sxx = Sxx[lo][hi], syy = Syy[lo][hi];
-            const double sxy = Sxy[i][j], syx = Sxy[j][i];
-            store_entry(p, ch * p.C + (s % p.C), nd[j], (int32_t)e,
-                        f * (sxx + cs * syy), f * (nu * sxy + cs * syx),
-                        f * (nu * syx + cs * sxy), f * (syy + cs * sxx));
+            double k[4];
+            el.block(i, j, k);
+            store_entry(p, ch * p.C + (s % p.C), el.nd[j], (int32_t)e, k);
         }
+    }
+}
+
+__device__ __forceinline__ void add_val(double* v, double x, bool atomic) {
+    if (atomic) atomicAdd(v, x);   // global_atomic_add_f64 (-munsafe-fp-atomics)
+    else *v += x;
+}
+
+template <int MODE, bool ATOMIC>
+__global__ __launch_bounds__(256) void k_scatter_2d(DevMesh m, Scatter sc, int32_t* misc) {
+    const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (t >= sc.count) return;
+    const int64_t e = sc.elist ? (int64_t)sc.elist[t] : t;
+    Elem2D el;
+    if (!el.load<MODE>(m, e, misc)) return;
+    const int32_t* mp = sc.map + e * 16;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= el.nn) continue;
+        const int64_t a = el.nd[i];
+        const int64_t rs = rp_at(sc.row_ptr, sc.i64, 2 * a);
+        const int64_t rowlen = rp_at(sc.row_ptr, sc.i64, 2 * a + 1) - rs;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= el.nn) continue;
+            double k[4];
+            el.block(i, j, k);
+            double* v = sc.val + rs + 2 * (int64_t)mp[i * 4 + j];
+            add_val(v, k[0], ATOMIC);
+            add_val(v + 1, k[1], ATOMIC);
+            add_val(v + rowlen, k[2], ATOMIC);
+            add_val(v + rowlen + 1, k[3], ATOMIC);
+        }
+    }
+}
+
+// Rank of every (row node i, column node j) pair of element e in the cached
+// pattern, by binary search of column b*dpn in DOF row a*dpn (the search of
+// the reference's binary_search, Acc:214-222, done once per pattern).
+template <typename RP>
+__global__ __launch_bounds__(256) void k_map(DevMesh m, int npe_map, const RP* rp, const int32_t* ci, int32_t* map,
+                                             int32_t* misc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = t / npe_map;
+    const int i = (int)(t % npe_map);
+    if (e >= m.n_elems) return;
+    const int kind = m.types ? (int)m.types[e] : m.uniform_type;
+    const int nn = kind == 0 ? 4 : kind == 1 ? 3 : 20;
+    if (i >= nn) return;
+    const int dpn = m.dim;
+    const int32_t* c = m.conn + e * m.npe;
+    const int64_t a = c[i];
+    const int64_t rs = (int64_t)rp[a * dpn], re = (int64_t)rp[a * dpn + 1];
+    for (int j = 0; j < nn; ++j) {
+        const int32_t target = c[j] * dpn;
+        int64_t lo = rs, hi = re - 1, pos = -1;
+        while (lo <= hi) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            const int32_t v = ci[mid];
+            if (v == target) { pos = mid; break; }
+            if (v < target) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        if (pos < 0) {
+            st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
+            pos = rs;
+        }
+        map[e * npe_map * npe_map + i * npe_map + j] = (int32_t)((pos - rs) / dpn);
     }
 }
 
@@ -163,6 +134,20 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = i; k < n; k += stride) p[k] = v;
+}
+
+__global__ void k_fill_f64(double* p, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n2 = n / 2;
+    double2* q = reinterpret_cast<double2*>(p);
+    for (int64_t k = i; k < n2; k += stride) q[k] = make_double2(v, v);
+    if (i == 0 && (n & 1)) p[n - 1] = v;
+}
+
+unsigned grid_for(int64_t n, int block) {
+    int64_t b = (n + block - 1) / block;
+    return (unsigned)(b < 1 ? 1 : b);
 }
 
 }  // namespace
@@ -176,16 +161,48 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
 }
 
+void launch_fill_f64(double* p, int64_t n, double v, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n / 2 + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
 void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, hipStream_t s) {
     if (m.n_elems <= 0) return;
     if (block != 64 && block != 128 && block != 256) block = 256;
-    const int64_t blocks = (m.n_elems + block - 1) / block;
-    if (mode == 0)
-        hipLaunchKernelGGL(k_insert_2d<0>, dim3((unsigned)blocks), dim3(block), 0, s, m, p);
-    else if (mode == 1)
-        hipLaunchKernelGGL(k_insert_2d<1>, dim3((unsigned)blocks), dim3(block), 0, s, m, p);
+    const unsigned g = grid_for(m.n_elems, block);
+    if (mode == 0) hipLaunchKernelGGL(k_insert_2d<0>, dim3(g), dim3(block), 0, s, m, p);
+    else if (mode == 1) hipLaunchKernelGGL(k_insert_2d<1>, dim3(g), dim3(block), 0, s, m, p);
+    else hipLaunchKernelGGL(k_insert_2d<2>, dim3(g), dim3(block), 0, s, m, p);
+}
+
+void launch_build_map(const DevMesh& m, int kind, const void* row_ptr, int i64, const int32_t* col_ind,
+                      int32_t* map, int32_t* misc, hipStream_t s) {
+    if (m.n_elems <= 0) return;
+    const int npe_map = kind == 3 ? 20 : 4;
+    const unsigned g = grid_for(m.n_elems * npe_map, 256);
+    if (i64)
+        hipLaunchKernelGGL(k_map<int64_t>, dim3(g), dim3(256), 0, s, m, npe_map,
+                           static_cast<const int64_t*>(row_ptr), col_ind, map, misc);
     else
-        hipLaunchKernelGGL(k_insert_2d<2>, dim3((unsigned)blocks), dim3(block), 0, s, m, p);
+        hipLaunchKernelGGL(k_map<int32_t>, dim3(g), dim3(256), 0, s, m, npe_map,
+                           static_cast<const int32_t*>(row_ptr), col_ind, map, misc);
+}
+
+void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic, int32_t* misc, hipStream_t s) {
+    if (sc.count <= 0) return;
+    const unsigned g = grid_for(sc.count, 256);
+#define ALLS_SC(MODE)                                                                              \
+    do {                                                                                           \
+        if (atomic) hipLaunchKernelGGL((k_scatter_2d<MODE, true>), dim3(g), dim3(256), 0, s, m, sc, misc);  \
+        else hipLaunchKernelGGL((k_scatter_2d<MODE, false>), dim3(g), dim3(256), 0, s, m, sc, misc);        \
+    } while (0)
+    if (kind == 0) ALLS_SC(0);
+    else if (kind == 1) ALLS_SC(1);
+    else ALLS_SC(2);
+#undef ALLS_SC
 }
 
 }  // namespace alls

@@ -79,8 +79,10 @@ template <> struct OutRow<float> {
     static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
 };
 
-template <typename T>
-__global__ __launch_bounds__(64) void k_insert_hex20(DevMesh m, ListPool p) {
+// EMIT 0: append to the A-LLS lists; 1: FP64-atomic scatter into a cached
+// pattern (value-only re-assembly); 2: plain "+=" scatter (one colour).
+template <typename T, int EMIT>
+__global__ __launch_bounds__(64) void k_hex20(DevMesh m, ListPool p, Scatter sc, int32_t* misc) {
     constexpr int LDS_T = (2 * KS * XS > 60 * MS) ? 2 * KS * XS : 60 * MS;
     __shared__ T lds[LDS_T];
     __shared__ double xyz[60];
@@ -88,8 +90,9 @@ __global__ __launch_bounds__(64) void k_insert_hex20(DevMesh m, ListPool p) {
     __shared__ int32_t posv[20];
     __shared__ int32_t bad;
 
-    const int64_t e = xcd_block(blockIdx.x, gridDim.x);
-    if (e >= m.n_elems) return;
+    const int64_t idx = xcd_block(blockIdx.x, gridDim.x);
+    if (idx >= (EMIT == 0 ? m.n_elems : sc.count)) return;
+    const int64_t e = (EMIT != 0 && sc.elist) ? (int64_t)sc.elist[idx] : idx;
     const int lane = threadIdx.x;
     if (lane == 0) bad = 0;
     __syncthreads();
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(64) void k_insert_hex20(DevMesh m, ListPool p) {
     }
     __syncthreads();
     if (bad || (m.types && m.types[e] != 2)) {
-        if (lane == 0) st_agent(&p.misc[MISC_ERR + ERR_INVALID_MESH], 1);
+        if (lane == 0) st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
         return;
     }
 
@@ -200,36 +203,58 @@ __global__ __launch_bounds__(64) void k_insert_hex20(DevMesh m, ListPool p) {
                 ++t;
             }
     }
-    // Reserve 20 slots in each of the element's 20 node rows.
-    if (lane < 20) {
-        const int32_t pos = list_reserve(p, nodes[lane], 20);
-        list_own_chunks(p, nodes[lane], pos, 20);
-        posv[lane] = pos;
-    }
-    __syncthreads();
-
     const double E = m.E_elem ? m.E_elem[e] : m.E;
     const double nu = m.nu;
     const T lam = (T)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
     const T mu = (T)(E / (2.0 * (1.0 + nu)));
+    if (EMIT == 0) {
+        // Reserve 20 slots in each of the element's 20 node rows.
+        if (lane < 20) {
+            const int32_t pos = list_reserve(p, nodes[lane], 20);
+            list_own_chunks(p, nodes[lane], pos, 20);
+            posv[lane] = pos;
+        }
+    }
+    __syncthreads();
+
     for (int pr = lane; pr < 400; pr += 64) {
         const int I = pr / 20, Jn = pr % 20;
-        const int32_t s = posv[I] + Jn;
-        const int64_t ch = list_chunk(p, nodes[I], s);
-        if (ch < 0) continue;
-        const int64_t idx = ch * p.C + (s % p.C);
         T mb[3][3];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) mb[r][c] = M[(3 * I + r) * MS + 3 * Jn + c];
         const T tr = mb[0][0] + mb[1][1] + mb[2][2];
-        T* dst = static_cast<T*>(p.vals) + idx * 9;
+        T kb[3][3];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) dst[r * 3 + c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
-        p.keys[idx] = make_int2(nodes[Jn], (int32_t)e);
+            for (int c = 0; c < 3; ++c) kb[r][c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
+        if (EMIT == 0) {
+            const int32_t s = posv[I] + Jn;
+            const int64_t ch = list_chunk(p, nodes[I], s);
+            if (ch < 0) continue;
+            const int64_t id = ch * p.C + (s % p.C);
+            T* dst = static_cast<T*>(p.vals) + id * 9;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
+            p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
+        } else {
+            const int64_t a = nodes[I];
+            const int64_t rs = rp_at(sc.row_ptr, sc.i64, 3 * a);
+            const int64_t rowlen = rp_at(sc.row_ptr, sc.i64, 3 * a + 1) - rs;
+            double* v = sc.val + rs + 3 * (int64_t)sc.map[e * 400 + pr];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double* q = v + r * rowlen + c;
+                    if (EMIT == 1) atomicAdd(q, (double)kb[r][c]);
+                    else *q += (double)kb[r][c];
+                }
+        }
     }
 }
 
@@ -239,10 +264,24 @@ namespace alls {
 
 void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, hipStream_t s) {
     if (m.n_elems <= 0) return;
+    const Scatter none{};
     if (f32)
-        hipLaunchKernelGGL(k_insert_hex20<float>, dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p);
+        hipLaunchKernelGGL((k_hex20<float, 0>), dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p, none, p.misc);
     else
-        hipLaunchKernelGGL(k_insert_hex20<double>, dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p);
+        hipLaunchKernelGGL((k_hex20<double, 0>), dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p, none, p.misc);
+}
+
+void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int atomic, int f32, int32_t* misc, hipStream_t s) {
+    if (sc.count <= 0) return;
+    const ListPool none{};
+    const dim3 g((unsigned)sc.count);
+    if (f32) {
+        if (atomic) hipLaunchKernelGGL((k_hex20<float, 1>), g, dim3(64), 0, s, m, none, sc, misc);
+        else hipLaunchKernelGGL((k_hex20<float, 2>), g, dim3(64), 0, s, m, none, sc, misc);
+    } else {
+        if (atomic) hipLaunchKernelGGL((k_hex20<double, 1>), g, dim3(64), 0, s, m, none, sc, misc);
+        else hipLaunchKernelGGL((k_hex20<double, 2>), g, dim3(64), 0, s, m, none, sc, misc);
+    }
 }
 
 }  // namespace alls

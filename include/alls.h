@@ -91,7 +91,8 @@ typedef struct {
     int64_t n_rows;
     int64_t nnz;
     int32_t row_ptr_is_i64;   /* row_ptr element type: int32 (0) or int64 (1) */
-    int32_t _pad;
+    int32_t pattern_id;       /* library bookkeeping: cached pattern whose row_ptr/col_ind
+                                 these buffers already hold (0 = none) */
     void*    row_ptr;         /* device [n_rows+1] */
     int32_t* col_ind;         /* device [nnz] */
     double*  val;             /* device [nnz] */
