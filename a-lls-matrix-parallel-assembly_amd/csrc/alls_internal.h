@@ -72,6 +72,8 @@ struct Scatter {
     int64_t count;
     int32_t i64;               // row_ptr is int64
     int32_t npe;               // map stride per node (4 in 2D, 20 for Hex20)
+    const int64_t* slot;       // Hex20 staged path: slot of each contribution
+    void* stage;               // Hex20 staged path: [slots][9] values (T)
 };
 
 #ifdef __HIPCC__
@@ -167,7 +169,18 @@ void launch_build_map(const DevMesh& m, int kind, const void* row_ptr, int i64, 
                       int32_t* map, int32_t* misc, hipStream_t s);
 // atomic = 1: FP64 atomics (all elements at once); 0: plain "+=" (one colour).
 void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic, int32_t* misc, hipStream_t s);
-void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int atomic, int f32, int32_t* misc, hipStream_t s);
+void launch_gather_2d(const DevMesh& m, int kind, const Scatter& sc, const int64_t* inc_off, const int32_t* inc_list,
+                      int32_t* misc, hipStream_t s);
+// Hex20 scatter: mode 1 = FP64 atomics, 0 = plain "+=" (one colour),
+// 3 = write each block to its staging slot (value-only path).
+void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int mode, int f32, int32_t* misc, hipStream_t s);
+void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* ci,
+                       const int64_t* inc_off, const int32_t* inc_list, int32_t* cnt, hipStream_t s);
+void launch_slot_map(int64_t n_elems, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
+                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int64_t* slot,
+                     hipStream_t s);
+void launch_pair_sum(int64_t n_nodes, const void* rp, int i64, const int64_t* goff, const void* stage, int f32,
+                     double* val, hipStream_t s);
 void launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
 // Colouring: node -> element incidence, JPL rounds, conflict check, buckets.
 void launch_incidence(const DevMesh& m, int32_t* cnt, int64_t* off, int32_t* cursor, int32_t* list, int64_t* tiles,

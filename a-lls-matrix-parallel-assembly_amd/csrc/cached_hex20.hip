@@ -1,0 +1,146 @@
+// cached_hex20.hip -- value-only Hex20 re-assembly on a cached pattern
+// (gfx950), without atomics, keys or sorting.
+//
+// At pattern build time every contribution (element e, row node I, column
+// node J) gets a fixed slot in a staging array grouped by node pair
+// (a = conn[e][I], rank r of b = conn[e][J] in row a), elements inside a group
+// in ascending id (the serial order of Acc:336):
+//   k_pair_count : contributions per node pair (elements sharing a and b);
+//   (scan)       : group offsets;
+//   k_slot_map   : slot of each contribution.
+// A repeat request then integrates each element (k_hex20 with EMIT 3 writes
+// its 400 blocks to their slots) and k_pair_sum adds every group and writes
+// the 3x3 block into the CSR rows.
+#include "alls_internal.h"
+
+namespace {
+
+// True when node b is one of element e's 20 nodes.
+__device__ __forceinline__ bool has_node(const int32_t* conn, int64_t e, int32_t b) {
+    const int32_t* c = conn + e * 20;
+    bool f = false;
+#pragma unroll
+    for (int i = 0; i < 20; ++i) f |= c[i] == b;
+    return f;
+}
+
+// Thread per node row a: for every rank r, the number of elements of a's
+// incidence list that also hold column node b_r.
+template <typename RP>
+__global__ __launch_bounds__(256) void k_pair_count(int64_t n_nodes, const int32_t* conn, const RP* rp,
+                                                    const int32_t* ci, const int64_t* inc_off,
+                                                    const int32_t* inc_list, int32_t* cnt) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_nodes) return;
+    const int64_t r0 = (int64_t)rp[3 * a];
+    const int64_t u = ((int64_t)rp[3 * a + 1] - r0) / 3;
+    const int64_t g0 = r0 / 9;
+    for (int64_t r = 0; r < u; ++r) {
+        const int32_t b = ci[r0 + 3 * r] / 3;
+        int32_t n = 0;
+        for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) n += has_node(conn, inc_list[k], b) ? 1 : 0;
+        cnt[g0 + r] = n;
+    }
+}
+
+// Thread per (element, row node I): slot of each of the 20 contributions.
+template <typename RP>
+__global__ __launch_bounds__(256) void k_slot_map(int64_t n_elems, const int32_t* conn, const RP* rp,
+                                                  const int32_t* rank_map, const int64_t* inc_off,
+                                                  const int32_t* inc_list, const int64_t* goff, int64_t* slot) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t e = t / 20;
+    const int I = (int)(t % 20);
+    if (e >= n_elems) return;
+    const int32_t* c = conn + e * 20;
+    const int64_t a = c[I];
+    const int64_t lo = inc_off[a];
+    int64_t k = lo;
+    while (k < inc_off[a + 1] && inc_list[k] != e) ++k;
+    if (k == inc_off[a + 1]) k = lo;   // not reachable for a valid incidence; keeps slots in bounds
+    const int64_t g0 = (int64_t)rp[3 * a] / 9;
+    for (int J = 0; J < 20; ++J) {
+        const int32_t b = c[J];
+        int64_t pos = 0;
+        for (int64_t q = lo; q < k; ++q) pos += has_node(conn, inc_list[q], b) ? 1 : 0;
+        slot[e * 400 + I * 20 + J] = goff[g0 + rank_map[e * 400 + I * 20 + J]] + pos;
+    }
+}
+
+// Wave per node row: sums each node pair's contributions in slot order and
+// writes the 3x3 block to DOF rows 3a..3a+2.
+template <typename T, typename RP>
+__global__ __launch_bounds__(256) void k_pair_sum(int64_t n_nodes, const RP* rp, const int64_t* goff, const T* stage,
+                                                  double* val) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t a = wave; a < n_nodes; a += nw) {
+        const int64_t r0 = (int64_t)rp[3 * a], r1 = (int64_t)rp[3 * a + 1], r2 = (int64_t)rp[3 * a + 2];
+        const int64_t u = (r1 - r0) / 3;
+        const int64_t g0 = r0 / 9;
+        for (int64_t r = lane; r < u; r += 64) {
+            double acc[9];
+#pragma unroll
+            for (int v = 0; v < 9; ++v) acc[v] = 0.0;
+            for (int64_t s = goff[g0 + r]; s < goff[g0 + r + 1]; ++s) {
+#pragma unroll
+                for (int v = 0; v < 9; ++v) acc[v] += (double)stage[s * 9 + v];
+            }
+            const int64_t rows[3] = {r0, r1, r2};
+#pragma unroll
+            for (int dr = 0; dr < 3; ++dr)
+#pragma unroll
+                for (int dc = 0; dc < 3; ++dc) val[rows[dr] + 3 * r + dc] = acc[dr * 3 + dc];
+        }
+    }
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+
+}  // namespace
+
+namespace alls {
+
+void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* ci,
+                       const int64_t* inc_off, const int32_t* inc_list, int32_t* cnt, hipStream_t s) {
+    if (n_nodes <= 0) return;
+    if (i64)
+        hipLaunchKernelGGL(k_pair_count<int64_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
+                           static_cast<const int64_t*>(rp), ci, inc_off, inc_list, cnt);
+    else
+        hipLaunchKernelGGL(k_pair_count<int32_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
+                           static_cast<const int32_t*>(rp), ci, inc_off, inc_list, cnt);
+}
+
+void launch_slot_map(int64_t n_elems, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
+                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int64_t* slot,
+                     hipStream_t s) {
+    if (n_elems <= 0) return;
+    if (i64)
+        hipLaunchKernelGGL(k_slot_map<int64_t>, dim3(grid_for(n_elems * 20)), dim3(256), 0, s, n_elems, conn,
+                           static_cast<const int64_t*>(rp), rank_map, inc_off, inc_list, goff, slot);
+    else
+        hipLaunchKernelGGL(k_slot_map<int32_t>, dim3(grid_for(n_elems * 20)), dim3(256), 0, s, n_elems, conn,
+                           static_cast<const int32_t*>(rp), rank_map, inc_off, inc_list, goff, slot);
+}
+
+void launch_pair_sum(int64_t n_nodes, const void* rp, int i64, const int64_t* goff, const void* stage, int f32,
+                     double* val, hipStream_t s) {
+    if (n_nodes <= 0) return;
+    int64_t blocks = (n_nodes + 3) / 4;
+    if (blocks > 262144) blocks = 262144;
+#define ALLS_PS(T, RP)                                                                                        \
+    hipLaunchKernelGGL((k_pair_sum<T, RP>), dim3((unsigned)blocks), dim3(256), 0, s, n_nodes,                 \
+                       static_cast<const RP*>(rp), goff, static_cast<const T*>(stage), val)
+    if (f32) {
+        if (i64) ALLS_PS(float, int64_t);
+        else ALLS_PS(float, int32_t);
+    } else {
+        if (i64) ALLS_PS(double, int64_t);
+        else ALLS_PS(double, int32_t);
+    }
+#undef ALLS_PS
+}
+
+}  // namespace alls

@@ -44,6 +44,23 @@ __global__ void k_inc_fill(DevMesh m, const int64_t* off, int32_t* cursor, int32
     }
 }
 
+// Sorts each node's incident elements by id (insertion sort; lists are short)
+// so that gathers visit elements in the serial assembly's order.
+__global__ void k_inc_sort(int64_t n_nodes, const int64_t* off, int32_t* list) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_nodes) return;
+    const int64_t lo = off[a], hi = off[a + 1];
+    for (int64_t i = lo + 1; i < hi; ++i) {
+        const int32_t v = list[i];
+        int64_t j = i - 1;
+        while (j >= lo && list[j] > v) {
+            list[j + 1] = list[j];
+            --j;
+        }
+        list[j + 1] = v;
+    }
+}
+
 // One JPL round.  misc[0] counts elements still uncoloured after the round,
 // misc[1] is the largest colour used.
 __global__ void k_jpl_round(DevMesh m, const int64_t* off, const int32_t* list, const int32_t* cin, int32_t* cout,
@@ -146,6 +163,7 @@ void launch_incidence(const DevMesh& m, int32_t* cnt, int64_t* off, int32_t* cur
     launch_scan_total(cnt, m.n_nodes, tiles, scan_tile_count(m.n_nodes), s);
     launch_scan_apply(cnt, m.n_nodes, tiles, 1, off, 1, s);
     if (m.n_elems > 0) hipLaunchKernelGGL(k_inc_fill, dim3(grid_for(m.n_elems)), dim3(256), 0, s, m, off, cursor, list);
+    if (m.n_nodes > 0) hipLaunchKernelGGL(k_inc_sort, dim3(grid_for(m.n_nodes)), dim3(256), 0, s, m.n_nodes, off, list);
 }
 
 void launch_jpl_round(const DevMesh& m, const int64_t* off, const int32_t* list, const int32_t* cin, int32_t* cout,

@@ -1,8 +1,9 @@
 // insert_2d.hip -- Q4 / T3 / mixed plane-stress kernels (gfx950):
 //   k_insert_2d  : element integration fused with A-LLS list insertion;
 //   k_map        : element -> CSR-slot map of a cached pattern;
-//   k_scatter_2d : value-only (FP64 atomics) or coloured (plain "+=")
-//                  scatter through that map.
+//   k_scatter_2d : coloured (plain "+=") or FP64-atomic scatter through
+//                  that map;
+//   k_gather_2d  : value-only re-assembly, one thread per node row.
 //
 // k_insert_2d replaces assembleGlobalStiffnessKernelALLS (Acc:394-468,
 // T3:227-290, Mix:248-312).  The reference walks each row's linked list per
@@ -12,7 +13,7 @@
 // reservation atomic per (element, row node) instead of a list walk and an
 // atomic per scalar.  Equal columns are merged by the per-row sort in
 // consolidation.  k_scatter_2d replaces the coloured kernels
-// (Acc:504-572, T3:316-363, Mix:341-390) and the value-only re-assembly.
+// (Acc:504-572, T3:316-363, Mix:341-390).
 #include "elem2d.h"
 
 using alls_dev::Elem2D;
@@ -91,6 +92,40 @@ __global__ __launch_bounds__(256) void k_scatter_2d(DevMesh m, Scatter sc, int32
             add_val(v + 1, k[1], ATOMIC);
             add_val(v + rowlen, k[2], ATOMIC);
             add_val(v + rowlen + 1, k[3], ATOMIC);
+        }
+    }
+}
+
+// Value-only re-assembly of a cached 2D pattern without atomics: one thread
+// owns node row a, walks the elements incident to a in ascending element id
+// (the serial assembly's order, Acc:336), recomputes each element's moments
+// and adds its blocks K_ij (i = local index of a) straight into the CSR rows
+// a*2 and a*2+1, which no other thread touches.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const int64_t* inc_off,
+                                                   const int32_t* inc_list, int32_t* misc) {
+    const int64_t a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (a >= m.n_nodes) return;
+    const int64_t r0 = rp_at(sc.row_ptr, sc.i64, 2 * a);
+    const int64_t r1 = rp_at(sc.row_ptr, sc.i64, 2 * a + 1);
+    const int64_t r2 = rp_at(sc.row_ptr, sc.i64, 2 * a + 2);
+    for (int64_t q = r0; q < r2; ++q) sc.val[q] = 0.0;
+    const int64_t rowlen = r1 - r0;
+    for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) {
+        const int64_t e = inc_list[k];
+        Elem2D el;
+        if (!el.load<MODE>(m, e, misc)) continue;
+        int i = 0;
+        while (i < el.nn - 1 && el.nd[i] != a) ++i;
+        const int32_t* mp = sc.map + e * 16 + i * 4;
+        for (int j = 0; j < el.nn; ++j) {
+            double kb[4];
+            el.block(i, j, kb);
+            double* v = sc.val + r0 + 2 * (int64_t)mp[j];
+            v[0] += kb[0];
+            v[1] += kb[1];
+            v[rowlen] += kb[2];
+            v[rowlen + 1] += kb[3];
         }
     }
 }
@@ -203,6 +238,15 @@ void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic
     else if (kind == 1) ALLS_SC(1);
     else ALLS_SC(2);
 #undef ALLS_SC
+}
+
+void launch_gather_2d(const DevMesh& m, int kind, const Scatter& sc, const int64_t* inc_off, const int32_t* inc_list,
+                      int32_t* misc, hipStream_t s) {
+    if (m.n_nodes <= 0) return;
+    const unsigned g = grid_for(m.n_nodes, 256);
+    if (kind == 0) hipLaunchKernelGGL(k_gather_2d<0>, dim3(g), dim3(256), 0, s, m, sc, inc_off, inc_list, misc);
+    else if (kind == 1) hipLaunchKernelGGL(k_gather_2d<1>, dim3(g), dim3(256), 0, s, m, sc, inc_off, inc_list, misc);
+    else hipLaunchKernelGGL(k_gather_2d<2>, dim3(g), dim3(256), 0, s, m, sc, inc_off, inc_list, misc);
 }
 
 }  // namespace alls

@@ -80,7 +80,8 @@ template <> struct OutRow<float> {
 };
 
 // EMIT 0: append to the A-LLS lists; 1: FP64-atomic scatter into a cached
-// pattern (value-only re-assembly); 2: plain "+=" scatter (one colour).
+// pattern; 2: plain "+=" scatter (one colour); 3: write each block to its
+// precomputed staging slot (value-only re-assembly, no atomics).
 template <typename T, int EMIT>
 __global__ __launch_bounds__(64) void k_hex20(DevMesh m, ListPool p, Scatter sc, int32_t* misc) {
     constexpr int LDS_T = (2 * KS * XS > 60 * MS) ? 2 * KS * XS : 60 * MS;
@@ -241,6 +242,12 @@ __global__ __launch_bounds__(64) void k_hex20(DevMesh m, ListPool p, Scatter sc,
 #pragma unroll
                 for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
             p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
+        } else if (EMIT == 3) {
+            T* dst = static_cast<T*>(sc.stage) + sc.slot[e * 400 + pr] * 9;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
         } else {
             const int64_t a = nodes[I];
             const int64_t rs = rp_at(sc.row_ptr, sc.i64, 3 * a);
@@ -271,17 +278,21 @@ void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, hipStream
         hipLaunchKernelGGL((k_hex20<double, 0>), dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p, none, p.misc);
 }
 
-void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int atomic, int f32, int32_t* misc, hipStream_t s) {
+void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int mode, int f32, int32_t* misc, hipStream_t s) {
     if (sc.count <= 0) return;
     const ListPool none{};
     const dim3 g((unsigned)sc.count);
+#define ALLS_HX(T, E) hipLaunchKernelGGL((k_hex20<T, E>), g, dim3(64), 0, s, m, none, sc, misc)
     if (f32) {
-        if (atomic) hipLaunchKernelGGL((k_hex20<float, 1>), g, dim3(64), 0, s, m, none, sc, misc);
-        else hipLaunchKernelGGL((k_hex20<float, 2>), g, dim3(64), 0, s, m, none, sc, misc);
+        if (mode == 1) ALLS_HX(float, 1);
+        else if (mode == 3) ALLS_HX(float, 3);
+        else ALLS_HX(float, 2);
     } else {
-        if (atomic) hipLaunchKernelGGL((k_hex20<double, 1>), g, dim3(64), 0, s, m, none, sc, misc);
-        else hipLaunchKernelGGL((k_hex20<double, 2>), g, dim3(64), 0, s, m, none, sc, misc);
+        if (mode == 1) ALLS_HX(double, 1);
+        else if (mode == 3) ALLS_HX(double, 3);
+        else ALLS_HX(double, 2);
     }
+#undef ALLS_HX
 }
 
 }  // namespace alls

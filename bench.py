@@ -171,6 +171,22 @@ def hex20_compare(ctx, n, steps):
                      "ms_write": round(st["ms_write"], 3), "nnz": st["nnz"]}
         v[name] = out.download()[2]
         out.free()
+    # Cached pattern: value-only re-assembly (staged, no atomics) and the
+    # coloured path with the 8-colour cell-parity colouring.
+    for name, prec in (("f64", alls.PREC_F64), ("f32_mfma", alls.PREC_F32_MFMA)):
+        pat = alls.Pattern(ctx, m, precision=prec, c_mesh=dm.c_mesh())
+        pat.out.free()
+        dt, out = time_calls(lambda o: pat.values(out=o), steps, 1)
+        res[name]["value_only_ms"] = round(dt * 1e3, 3)
+        if name == "f64":
+            col = pat.color_from_host(m.structured_color)
+            dtc, out2 = time_calls(lambda o: pat.colored(col, out=o), steps, 1)
+            res["colored_8color_f64_ms"] = round(dtc * 1e3, 3)
+            res["alls_full_vs_colored_speedup_f64"] = round(dtc * 1e3 / res["f64"]["ms_per_step"], 3)
+            out2.free()
+            col.free()
+        out.free()
+        pat.free()
     dm.free()
     res["speedup_f32_vs_f64"] = round(res["f64"]["ms_per_step"] / res["f32_mfma"]["ms_per_step"], 3)
     if v["f64"] is not None:
