@@ -488,3 +488,67 @@ class DeviceMesh:
             self.free()
         except Exception:
             pass
+
+
+# ----------------------------------------------------------------- server --
+class Server:
+    """Continuous-batching assembly service (alls_server_*)."""
+
+    def __init__(self, device=0, n_streams=4, hbm_budget_bytes=0, pattern_cache_entries=64):
+        cfg = ServerConfig()
+        cfg.device, cfg.n_streams = device, n_streams
+        cfg.hbm_budget_bytes, cfg.pattern_cache_entries = hbm_budget_bytes, pattern_cache_entries
+        h = vp()
+        check(lib().alls_server_create(C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, mesh, key=0, precision=PREC_F64, c_mesh=None):
+        """Enqueue a request; `mesh` (its arrays) is kept alive until wait()."""
+        o = Opts()
+        o.precision = precision
+        m = c_mesh if c_mesh is not None else mesh.c_mesh()
+        t = C.c_int64()
+        check(lib().alls_server_submit(self.handle, C.byref(m), C.c_uint64(key), C.byref(o), C.byref(t)))
+        self._keep[t.value] = (mesh, m)
+        return t.value
+
+    def wait(self, ticket, keep_output=True):
+        """Returns (Csr or None, stats dict); the Csr must be released()."""
+        out = Csr()
+        st = RequestStats()
+        rc = lib().alls_server_wait(self.handle, ticket, C.byref(out) if keep_output else None, C.byref(st))
+        self._keep.pop(ticket, None)
+        stats = {k: getattr(st, k) for k, _ in RequestStats._fields_}
+        if rc != OK:
+            raise AllsError(rc, f"request {ticket}")
+        return (out if keep_output else None), stats
+
+    def download(self, csr):
+        """Host copy of a server-owned CSR (uses a temporary context)."""
+        rp = np.empty(csr.n_rows + 1, dtype=np.int64)
+        ci = np.empty(max(csr.nnz, 1), dtype=np.int32)
+        v = np.empty(max(csr.nnz, 1), dtype=np.float64)
+        with Context(0) as ctx:
+            check(lib().alls_csr_download(ctx.handle, C.byref(csr), rp.ctypes.data, ci.ctypes.data, v.ctypes.data))
+        return rp, ci[:csr.nnz], v[:csr.nnz]
+
+    def release(self, csr):
+        lib().alls_server_release(self.handle, C.byref(csr))
+
+    def close(self):
+        if self.handle:
+            lib().alls_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

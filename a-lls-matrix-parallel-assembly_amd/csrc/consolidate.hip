@@ -11,7 +11,8 @@
 //           write col_ind / val of the dpn DOF rows of the node.
 // Rows of <= G entries use G lanes and a register bitonic sort through
 // cross-lane shuffles.  Longer rows (Hex20, or unstructured outliers) use a
-// wave per row with the sort staged in LDS.  Sorting by (column, element id)
+// wave per row: an LDS hash set for the count, a sort staged in LDS for the
+// write.  Sorting by (column, element id)
 // makes the summation order, and so the values, independent of the order in
 // which atomics appended the entries.
 #include "alls_internal.h"
@@ -212,6 +213,42 @@ __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list
     }
 }
 
+// Unique-column count of long rows with an LDS hash set (one 64-thread block
+// per row at a time, `ts` table slots, ts >= 2 * ALLS_DIR * C).  Counting
+// needs no order, so no sort: each entry inserts its column with one LDS
+// compare-and-swap (linear probing).
+__global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_t* list, int32_t* u, int ts) {
+    extern __shared__ int32_t table[];
+    const int lane = threadIdx.x;
+    const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
+    const int shift = 32 - (31 - __clz(ts));
+    for (int64_t r = blockIdx.x; r < count; r += gridDim.x) {
+        const int64_t a = list ? (int64_t)list[r] : r;
+        const int32_t* h = p.hdr + a * ALLS_DIR;
+        const int32_t n = min(h[0], ALLS_DIR * p.C);
+        for (int i = lane; i < ts; i += 64) table[i] = -1;
+        __syncthreads();
+        int32_t mine = 0;
+        for (int i = lane; i < n; i += 64) {
+            const int32_t col = p.keys[entry_index(p, a, h, i)].x;
+            uint32_t slot = ((uint32_t)col * 0x9E3779B1u) >> shift;
+            for (int probe = 0; probe < ts; ++probe) {
+                const int32_t old = atomicCAS(&table[slot], -1, col);
+                if (old == -1) {
+                    ++mine;
+                    break;
+                }
+                if (old == col) break;
+                slot = (slot + 1) & (uint32_t)(ts - 1);
+            }
+        }
+        int32_t total;
+        wave_excl_scan(mine, &total);
+        if (lane == 0) u[a] = total;
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------- scan ---
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER = 16;
@@ -302,14 +339,19 @@ void rows_dispatch(const ListPool& p, int G, const int32_t* big_list, int32_t* u
         else
             hipLaunchKernelGGL((k_rows_small<64, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn);
     }
-    // LDS kernel: over the big-row list (G > 0) or over every row (G == 0).
+    // Long rows: over the big-row list (G > 0) or over every row (G == 0).
     int cap = 1;
     while (cap < ALLS_DIR * p.C) cap <<= 1;
-    const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(uint16_t));
-    const int64_t rows = G > 0 ? p.n_nodes : p.n_nodes;  // upper bound of the list length
-    int64_t grid = G > 0 ? 2048 : rows;
+    int64_t grid = G > 0 ? 2048 : p.n_nodes;
     if (grid > 262144) grid = 262144;
     if (grid < 1) grid = 1;
+    if (!WRITE) {
+        const int ts = 2 * cap;
+        hipLaunchKernelGGL(k_rows_hash_count, dim3((unsigned)grid), dim3(64), (size_t)ts * sizeof(int32_t), s, p,
+                           G > 0 ? big_list : nullptr, u, ts);
+        return;
+    }
+    const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(uint16_t));
     hipLaunchKernelGGL((k_rows_big<B, VT, WRITE, RP>), dim3((unsigned)grid), dim3(64), lds, s, p,
                        G > 0 ? big_list : nullptr, u, rp, ci, v, dpn, cap);
 }

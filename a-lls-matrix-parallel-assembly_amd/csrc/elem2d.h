@@ -39,17 +39,24 @@ struct Elem2D {
         } else {
             nd[0] = c[0]; nd[1] = c[1]; nd[2] = c[2]; nd[3] = nn == 4 ? c[3] : 0;
         }
+        // Loops run over the 4 slots with a predicate (never up to a runtime
+        // bound) so that the arrays stay in registers.
         bool ok = true;
-        for (int i = 0; i < nn; ++i) ok &= (nd[i] >= 0) & (nd[i] < m.n_nodes);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i < nn) ok &= (nd[i] >= 0) & (nd[i] < m.n_nodes);
         if (!ok) {
             st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
             return false;
         }
-        double x[4], y[4];
-        for (int i = 0; i < nn; ++i) {
-            const double2 q = *reinterpret_cast<const double2*>(m.coords + 2 * (int64_t)nd[i]);
-            x[i] = q.x;
-            y[i] = q.y;
+        double x[4] = {0, 0, 0, 0}, y[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i < nn) {
+                const double2 q = *reinterpret_cast<const double2*>(m.coords + 2 * (int64_t)nd[i]);
+                x[i] = q.x;
+                y[i] = q.y;
+            }
         }
         const double E = m.E_elem ? m.E_elem[e] : m.E;
         nu = m.nu;

@@ -69,8 +69,9 @@ alls_status hip_fail(hipError_t e, const char* what, std::string* err);
 // Pattern cache internals shared with the server (pattern.cpp).
 alls_status pattern_build(Workspace& ws, const alls_mesh* mesh, const alls_opts& o, hipStream_t s, int device,
                           alls_pattern** out_pat, alls_csr* out, alls_stats* st, std::string* err);
-alls_status pattern_values(Workspace& ws, alls_pattern* p, const double* E_dev, alls_csr* out, hipStream_t s,
-                           alls_stats* st, std::string* err);
+// E_dev (device, per element) or, when null, the scalar E_scalar.
+alls_status pattern_values(Workspace& ws, alls_pattern* p, const double* E_dev, double E_scalar, alls_csr* out,
+                           hipStream_t s, alls_stats* st, std::string* err);
 void pattern_destroy(alls_pattern* p);
 int64_t pattern_bytes(const alls_pattern* p);
 

@@ -115,17 +115,23 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
         const int64_t e = inc_list[k];
         Elem2D el;
         if (!el.load<MODE>(m, e, misc)) continue;
-        int i = 0;
-        while (i < el.nn - 1 && el.nd[i] != a) ++i;
-        const int32_t* mp = sc.map + e * 16 + i * 4;
-        for (int j = 0; j < el.nn; ++j) {
-            double kb[4];
-            el.block(i, j, kb);
-            double* v = sc.val + r0 + 2 * (int64_t)mp[j];
-            v[0] += kb[0];
-            v[1] += kb[1];
-            v[rowlen] += kb[2];
-            v[rowlen + 1] += kb[3];
+        // Static loops with a predicate on the local index of a: the moment
+        // arrays are indexed by constants only and stay in registers.
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i >= el.nn || el.nd[i] != a) continue;
+            const int32_t* mp = sc.map + e * 16 + i * 4;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j >= el.nn) continue;
+                double kb[4];
+                el.block(i, j, kb);
+                double* v = sc.val + r0 + 2 * (int64_t)mp[j];
+                v[0] += kb[0];
+                v[1] += kb[1];
+                v[rowlen] += kb[2];
+                v[rowlen + 1] += kb[3];
+            }
         }
     }
 }

@@ -265,14 +265,15 @@ alls_status pattern_build(Workspace& ws, const alls_mesh* mesh, const alls_opts&
 }
 
 // Value-only assembly on stream s with workspace scratch for flags/events.
-alls_status pattern_values(Workspace& ws, alls_pattern* p, const double* E_dev, alls_csr* out, hipStream_t s,
-                           alls_stats* st, std::string* err) {
+alls_status pattern_values(Workspace& ws, alls_pattern* p, const double* E_dev, double E_scalar, alls_csr* out,
+                           hipStream_t s, alls_stats* st, std::string* err) {
     HIP_TRY(ws.init(), "workspace init");
     alls_status rc = pattern_into(p, out, s, err);
     if (rc != ALLS_OK) return rc;
     HIP_TRY(ws.misc.ensure(MISC_SIZE * sizeof(int32_t)), "alloc misc");
     DevMesh dm = p->dm;
     dm.E_elem = E_dev;
+    dm.E = E_scalar;
     Scatter sc = make_scatter(p, out);
     sc.count = dm.n_elems;
     HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
@@ -344,7 +345,7 @@ alls_status alls_assemble_values(alls_ctx* ctx, const alls_pattern* cpat, const 
     alls_status rc = stage_E(ctx, p, E_per_elem, mem, &E, &ctx->err);
     if (rc != ALLS_OK) return rc;
     alls_stats st{};
-    rc = pattern_values(ctx->ws, p, E, out, ctx->stream, &st, &ctx->err);
+    rc = pattern_values(ctx->ws, p, E, p->dm.E, out, ctx->stream, &st, &ctx->err);
     if (rc == ALLS_OK) ctx->stats = st;
     return rc;
 }

@@ -1,0 +1,363 @@
+// server.cpp -- continuous-batching assembly service on one GPU.
+//
+// The reference has no serving layer; its drivers run one mesh at a time on
+// the legacy stream (Acc:750-893).  Here n worker threads each own a HIP
+// stream and a workspace.  A request is admitted when a worker is idle and its
+// HBM footprint fits the remaining budget, so requests that arrive while
+// others run start as soon as a slot and memory free up, and their kernels
+// overlap on the device.  Requests that carry a mesh key keep that mesh's
+// pattern on the device (LRU-evicted within the budget).  A repeat request
+// with a new E field runs only the value-only path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "host_internal.h"
+
+using namespace alls;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+    int64_t ticket = 0;
+    alls_mesh mesh{};
+    uint64_t key = 0;
+    alls_opts opts{};
+    int64_t footprint = 0;
+    int64_t pattern_bytes = 0;      // extra bytes when this request builds a cached pattern
+    Clock::time_point t_submit, t_admit, t_done;
+    alls_status status = ALLS_OK;
+    bool done = false;
+    bool cache_hit = false;
+    alls_csr out{};
+    std::string err;
+};
+
+struct CacheEntry {
+    alls_pattern* pat = nullptr;
+    int64_t bytes = 0;
+    int refs = 0;
+    uint64_t last_use = 0;
+    int precision = 0;
+    int64_t n_elems = 0, n_nodes = 0;
+};
+
+// Upper-bound device bytes of one assembly (list pool + scratch + output).
+int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pattern, int64_t* pattern_bytes) {
+    const int64_t B = mi.dpn * mi.dpn;
+    const int64_t vsz = mi.f32 ? 4 : 8;
+    const int64_t ent = mi.entries_bound;
+    const int64_t C = mi.kind == 3 ? 80 : mi.kind == 0 ? 16 : 24;
+    const int64_t chunks = m->n_nodes + ent / C + 1;
+    const int64_t n_rows = m->n_nodes * mi.dpn;
+    const int64_t nnz_bound = ent * B;
+    int64_t scratch = chunks * C * (8 + B * vsz) + m->n_nodes * (32 + 8) + 1024;
+    int64_t inputs = m->n_nodes * m->dim * 8 + m->n_elems * (m->nodes_per_elem * 4 + 9);
+    int64_t out = (n_rows + 1) * 8 + nnz_bound * 12;
+    int64_t pat = 0;
+    if (with_pattern) {
+        const int64_t npm = mi.kind == 3 ? 20 : 4;
+        pat = inputs + (n_rows + 1) * 8 + nnz_bound * 4 + m->n_elems * npm * npm * 4 + m->n_nodes * 24 +
+              m->n_elems * mi.npe_max * 4;
+        if (mi.kind == 3) pat += (nnz_bound / 9 + 1) * 12 + m->n_elems * 400 * 8;
+    }
+    if (pattern_bytes) *pattern_bytes = pat;
+    return scratch + inputs + out + pat;
+}
+
+}  // namespace
+
+struct alls_server {
+    alls_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::unordered_map<int64_t, std::shared_ptr<Request>> reqs;
+    std::unordered_map<uint64_t, CacheEntry> cache;
+    std::unordered_map<uint64_t, int> building;   // keys whose pattern is being built
+    int64_t budget = 0, in_use = 0;
+    int64_t next_ticket = 1;
+    uint64_t use_clock = 0;
+    bool stop = false;
+    std::vector<std::thread> workers;
+    std::vector<hipStream_t> streams;
+    std::vector<std::unique_ptr<Workspace>> ws;
+
+    int64_t cache_bytes() const {
+        int64_t b = 0;
+        for (auto& kv : cache) b += kv.second.bytes;
+        return b;
+    }
+    // Frees least-recently used idle patterns until `need` bytes fit.
+    void evict_for(int64_t need) {
+        while (in_use + cache_bytes() + need > budget) {
+            auto victim = cache.end();
+            for (auto it = cache.begin(); it != cache.end(); ++it)
+                if (it->second.refs == 0 && (victim == cache.end() || it->second.last_use < victim->second.last_use))
+                    victim = it;
+            if (victim == cache.end()) return;
+            pattern_destroy(victim->second.pat);
+            cache.erase(victim);
+        }
+    }
+
+    void worker(int slot) {
+        (void)hipSetDevice(cfg.device);
+        hipStream_t s = streams[slot];
+        Workspace& w = *ws[slot];
+        for (;;) {
+            std::shared_ptr<Request> r;
+            bool hit = false, build = false;
+            CacheEntry* ce = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                for (;;) {
+                    if (stop) return;
+                    // First queued request whose footprint fits (FIFO with backfill).
+                    for (auto it = queue.begin(); it != queue.end() && !r; ++it) {
+                        auto& q = *it;
+                        auto c = q->key ? cache.find(q->key) : cache.end();
+                        const bool can_hit = c != cache.end() && c->second.precision == q->opts.precision &&
+                                             c->second.n_elems == q->mesh.n_elems &&
+                                             c->second.n_nodes == q->mesh.n_nodes;
+                        const bool can_build = q->key && !can_hit && !building.count(q->key) &&
+                                               (c == cache.end() || c->second.refs == 0);
+                        const int64_t need = can_build ? q->footprint : q->footprint - q->pattern_bytes;
+                        if (can_hit) c->second.refs++;   // pinned while we make room
+                        int64_t evictable = 0;
+                        for (auto& kv : cache)
+                            if (kv.second.refs == 0) evictable += kv.second.bytes;
+                        if (in_use + cache_bytes() - evictable + need <= budget) {
+                            evict_for(need);
+                            auto stale = can_build ? cache.find(q->key) : cache.end();
+                            if (stale != cache.end()) {   // idle entry of another mesh under this key
+                                pattern_destroy(stale->second.pat);
+                                cache.erase(stale);
+                            }
+                            r = q;
+                            queue.erase(it);
+                            hit = can_hit;
+                            build = can_build;
+                            if (hit) {
+                                ce = &cache[r->key];
+                                ce->last_use = ++use_clock;
+                            }
+                            if (build) building[r->key] = 1;
+                            r->footprint = need;
+                            in_use += need;
+                            break;
+                        }
+                        if (can_hit) c->second.refs--;
+                    }
+                    if (r) break;
+                    cv_work.wait(lk);
+                }
+            }
+            r->t_admit = Clock::now();
+            alls_stats st{};
+            alls_status rc;
+            if (hit) {
+                // New material field on the cached pattern: value-only path.
+                const double* E = r->mesh.E_per_elem;
+                rc = ALLS_OK;
+                if (E && r->mesh.mem == ALLS_MEM_HOST) {
+                    hipError_t e = w.E.ensure((size_t)std::max<int64_t>(r->mesh.n_elems, 1) * sizeof(double));
+                    if (e == hipSuccess)
+                        e = hipMemcpyAsync(w.E.p, E, (size_t)r->mesh.n_elems * sizeof(double), hipMemcpyHostToDevice, s);
+                    if (e != hipSuccess) rc = hip_fail(e, "stage E", &r->err);
+                    E = w.E.as<double>();
+                }
+                if (rc == ALLS_OK) rc = pattern_values(w, ce->pat, E, r->mesh.E, &r->out, s, &st, &r->err);
+            } else if (build) {
+                alls_pattern* p = nullptr;
+                rc = pattern_build(w, &r->mesh, r->opts, s, cfg.device, &p, &r->out, &st, &r->err);
+                std::lock_guard<std::mutex> lk(mu);
+                building.erase(r->key);
+                if (rc == ALLS_OK) {
+                    CacheEntry e;
+                    e.pat = p;
+                    e.bytes = pattern_bytes(p);
+                    e.last_use = ++use_clock;
+                    e.precision = r->opts.precision;
+                    e.n_elems = r->mesh.n_elems;
+                    e.n_nodes = r->mesh.n_nodes;
+                    if ((int)cache.size() >= std::max(1, cfg.pattern_cache_entries)) {
+                        auto victim = cache.end();
+                        for (auto it = cache.begin(); it != cache.end(); ++it)
+                            if (it->second.refs == 0 &&
+                                (victim == cache.end() || it->second.last_use < victim->second.last_use))
+                                victim = it;
+                        if (victim != cache.end()) {
+                            pattern_destroy(victim->second.pat);
+                            cache.erase(victim);
+                        }
+                    }
+                    cache[r->key] = e;
+                }
+            } else {
+                MeshInfo mi;
+                DevMesh dm;
+                rc = classify_mesh(&r->mesh, r->opts, &mi, &r->err);
+                if (rc == ALLS_OK) rc = stage_mesh(w, &r->mesh, mi, &dm, s, &r->err);
+                if (rc == ALLS_OK) rc = run_alls(w, dm, mi, r->opts, &r->out, s, &st, &r->err);
+            }
+            // Scratch above 1 GiB is returned so idle slots do not pin HBM.
+            for (DevBuf* b : {&w.keys, &w.vals})
+                if (b->bytes > (size_t(1) << 30)) b->release();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                r->t_done = Clock::now();
+                r->status = rc;
+                r->cache_hit = hit;
+                r->done = true;
+                if (hit) ce->refs--;
+                // The output stays charged until the caller releases it.
+                const int64_t outb = r->out.cap_nnz * 12 + r->out.cap_rows * 8;
+                in_use -= r->footprint;
+                r->footprint = rc == ALLS_OK ? outb : 0;
+                in_use += r->footprint;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+alls_status alls_server_create(const alls_server_config* cfg, alls_server** out) {
+    if (!cfg || !out) return ALLS_E_INVALID;
+    *out = nullptr;
+    if (alls_device_count() <= cfg->device || cfg->device < 0) return ALLS_E_NO_DEVICE;
+    if (hipSetDevice(cfg->device) != hipSuccess) return ALLS_E_HIP;
+    auto* s = new alls_server();
+    s->cfg = *cfg;
+    const int n = std::min(std::max(cfg->n_streams, 1), 16);
+    s->cfg.n_streams = n;
+    if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 64;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        delete s;
+        return ALLS_E_HIP;
+    }
+    s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)(0.8 * (double)free_b);
+    for (int i = 0; i < n; ++i) {
+        hipStream_t st;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            alls_server_destroy(s);
+            return ALLS_E_HIP;
+        }
+        s->streams.push_back(st);
+        s->ws.emplace_back(new Workspace());
+        if (s->ws.back()->init() != hipSuccess) {
+            alls_server_destroy(s);
+            return ALLS_E_HIP;
+        }
+    }
+    for (int i = 0; i < n; ++i) s->workers.emplace_back(&alls_server::worker, s, i);
+    *out = s;
+    return ALLS_OK;
+}
+
+alls_status alls_server_submit(alls_server* s, const alls_mesh* mesh, uint64_t key, const alls_opts* opts,
+                               int64_t* ticket) {
+    if (!s || !mesh || !ticket) return ALLS_E_INVALID;
+    auto r = std::make_shared<Request>();
+    r->mesh = *mesh;
+    r->key = key;
+    if (opts) r->opts = *opts;
+    MeshInfo mi;
+    std::string err;
+    alls_status rc = classify_mesh(mesh, r->opts, &mi, &err);
+    if (rc != ALLS_OK) return rc;
+    r->footprint = estimate_footprint(mesh, mi, key != 0, &r->pattern_bytes);
+    if (r->footprint - r->pattern_bytes > s->budget) return ALLS_E_BUSY;
+    if (r->footprint > s->budget) r->key = 0;   // fits only without caching its pattern
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->ticket = s->next_ticket++;
+        s->reqs[r->ticket] = r;
+        s->queue.push_back(r);
+    }
+    s->cv_work.notify_all();
+    *ticket = r->ticket;
+    return ALLS_OK;
+}
+
+alls_status alls_server_wait(alls_server* s, int64_t ticket, alls_csr* out, alls_request_stats* st) {
+    if (!s) return ALLS_E_INVALID;
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return ALLS_E_INVALID;
+        r = it->second;
+        s->cv_done.wait(lk, [&] { return r->done; });
+        s->reqs.erase(ticket);
+    }
+    if (st) {
+        st->queue_ms = ms_between(r->t_submit, r->t_admit);
+        st->run_ms = ms_between(r->t_admit, r->t_done);
+        st->latency_ms = ms_between(r->t_submit, r->t_done);
+        st->nnz = r->out.nnz;
+        st->cache_hit = r->cache_hit ? 1 : 0;
+        st->status = r->status;
+    }
+    if (out) {
+        *out = r->out;
+        std::memset(&r->out, 0, sizeof(r->out));
+    } else {
+        free_csr(&r->out);
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->in_use -= r->footprint;
+        s->cv_work.notify_all();
+    }
+    return r->status;
+}
+
+void alls_server_release(alls_server* s, alls_csr* csr) {
+    if (!s || !csr) return;
+    const int64_t b = csr->cap_nnz * 12 + csr->cap_rows * 8;
+    (void)hipSetDevice(s->cfg.device);
+    free_csr(csr);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->in_use -= b;
+        if (s->in_use < 0) s->in_use = 0;
+    }
+    s->cv_work.notify_all();
+}
+
+void alls_server_destroy(alls_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto& t : s->workers)
+        if (t.joinable()) t.join();
+    (void)hipSetDevice(s->cfg.device);
+    for (auto& kv : s->reqs) free_csr(&kv.second->out);
+    for (auto& kv : s->cache) pattern_destroy(kv.second.pat);
+    for (auto& w : s->ws) w->destroy();
+    for (auto st : s->streams) (void)hipStreamDestroy(st);
+    delete s;
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+"""Served-mix workload driver: Poisson arrivals of assembly requests.
+
+Request i draws its element type uniformly from {Q4, T3, Hex20} and its size
+log-uniformly in [min_elems, max_elems]; with probability `reuse` it repeats
+an earlier mesh (same mesh key, so the server's cached pattern applies) with
+a fresh lognormal(0, 0.5) E field.  All meshes and E fields are generated
+before the clock starts.  Latency is submit -> completion as the server
+measures it; throughput is total elements over the makespan.
+"""
+import math
+import threading
+import time
+
+import numpy as np
+
+import alls
+
+
+def _mesh_for(kind, n_elems, seed):
+    if kind == alls.HEX20:
+        n = max(1, round(n_elems ** (1.0 / 3.0)))
+        return alls.generate(alls.GEN_HEX20, n, n, n, jitter=0.15, seed=seed)
+    if kind == alls.T3:
+        n = max(1, round(math.sqrt(n_elems / 2.0)))
+        return alls.generate(alls.GEN_T3, n, n, jitter=0.3, seed=seed, random_diagonal=1, permute_nodes=1)
+    n = max(1, round(math.sqrt(n_elems)))
+    return alls.generate(alls.GEN_Q4, n, n, jitter=0.2, seed=seed)
+
+
+def build_requests(n_requests, seed=0xA11500C5, min_elems=1e3, max_elems=1e6, reuse=0.6):
+    rng = np.random.default_rng(seed)
+    meshes, reqs = [], []
+    for i in range(n_requests):
+        if meshes and rng.random() < reuse:
+            mid = int(rng.integers(len(meshes)))
+        else:
+            kind = int(rng.integers(3))
+            size = math.exp(rng.uniform(math.log(min_elems), math.log(max_elems)))
+            meshes.append(_mesh_for(kind, size, seed + 1 + len(meshes)))
+            mid = len(meshes) - 1
+        m = meshes[mid]
+        E = np.exp(0.5 * rng.standard_normal(m.n_elems))
+        reqs.append((mid, m.with_E(E)))
+    return meshes, reqs
+
+
+def expected_service_ms(m):
+    """Rough single-stream service time used only to set the arrival rate."""
+    per = {alls.Q4: 1.2e-6, alls.T3: 1.0e-6, alls.HEX20: 6.0e-5}
+    kind = alls.HEX20 if m.dim == 3 else m.uniform_type
+    return 0.05 + m.n_elems * per[kind]
+
+
+def run(server, reqs, rate_per_s, seed=1):
+    """Submits reqs at Poisson arrival times; returns per-request stats."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_per_s, size=len(reqs))
+    arrivals = np.cumsum(gaps)
+    tickets = [None] * len(reqs)
+    stats = [None] * len(reqs)
+    ready = threading.Semaphore(0)
+    errors = []
+
+    def submitter():
+        t0 = time.perf_counter()
+        for i, (mid, m) in enumerate(reqs):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            try:
+                tickets[i] = server.submit(m, key=mid + 1)
+            except Exception as e:   # refused request: record and go on
+                errors.append((i, str(e)))
+            ready.release()
+
+    th = threading.Thread(target=submitter)
+    t_start = time.perf_counter()
+    th.start()
+    for i in range(len(reqs)):
+        ready.acquire()
+        if tickets[i] is None:
+            continue
+        out, st = server.wait(tickets[i], keep_output=False)
+        stats[i] = st
+    th.join()
+    makespan = time.perf_counter() - t_start
+    return stats, makespan, errors
+
+
+def summarize(reqs, stats, makespan, errors):
+    lat = np.array([s["latency_ms"] for s in stats if s is not None])
+    q = np.array([s["queue_ms"] for s in stats if s is not None])
+    hits = sum(1 for s in stats if s is not None and s["cache_hit"])
+    elems = sum(m.n_elems for (_, m), s in zip(reqs, stats) if s is not None)
+    return {"requests": len(reqs), "completed": int(lat.size), "refused": len(errors),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(float(lat.mean()), 3), "queue_p99_ms": round(float(np.percentile(q, 99)), 3),
+            "cache_hits": hits, "elements": int(elems), "makespan_s": round(makespan, 3),
+            "elements_per_s": round(elems / makespan, 1)}

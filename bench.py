@@ -195,6 +195,25 @@ def hex20_compare(ctx, n, steps):
     return res
 
 
+def served_mix(n_requests):
+    """Poisson mix of Q4/T3/Hex20 requests (1e3-1e6 elements, 60% mesh reuse)."""
+    import alls
+    import serve_mix
+    t0 = time.perf_counter()
+    meshes, reqs = serve_mix.build_requests(n_requests)
+    t_gen = time.perf_counter() - t0
+    busy_s = sum(serve_mix.expected_service_ms(m) for _, m in reqs) / 1e3
+    rate = 0.75 * n_requests / busy_s
+    with alls.Server(n_streams=4) as srv:
+        stats, makespan, errors = serve_mix.run(srv, reqs, rate)
+    res = serve_mix.summarize(reqs, stats, makespan, errors)
+    res.update({"arrival_rate_per_s": round(rate, 1), "unique_meshes": len(meshes),
+                "generation_s": round(t_gen, 1), "streams": 4,
+                "workload": "Poisson arrivals; type uniform over {Q4, T3, Hex20}; size log-uniform 1e3-1e6 "
+                            "elements; 60% reuse a seen mesh with a new lognormal(0,0.5) E (cached pattern)"})
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +224,7 @@ def main():
     ap.add_argument("--hex-n", type=int, default=80)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built.",
@@ -258,6 +278,12 @@ def main():
             extra["hex20"] = hex20_compare(ctx, a.hex_n, 3)
         except Exception as e:  # report, do not hide the main line
             extra["hex20"] = {"error": str(e)}
+    if not a.no_extra and a.serve_requests > 0:
+        try:
+            log(f"served mix: {a.serve_requests} requests")
+            extra["served_mix"] = served_mix(a.serve_requests)
+        except Exception as e:
+            extra["served_mix"] = {"error": str(e)}
     if extra:
         res["extra"] = extra
     ctx.close()
