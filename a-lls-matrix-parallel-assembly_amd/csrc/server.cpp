@@ -4,8 +4,8 @@
 // the legacy stream (Acc:750-893).  Here n worker threads each own a HIP
 // stream and a workspace.  A request is admitted when a worker is idle and its
 // HBM footprint fits the remaining budget, so requests that arrive while
-// others run start as soon as a slot and memory free up, and their kernels
-// overlap on the device.  Requests that carry a mesh key keep that mesh's
+// others run start (in arrival order) as soon as a slot and memory free up,
+// and their kernels overlap on the device.  Requests that carry a mesh key keep that mesh's
 // pattern on the device (LRU-evicted within the budget).  A repeat request
 // with a new E field runs only the value-only path.
 #include <hip/hip_runtime.h>
@@ -127,8 +127,11 @@ struct alls_server {
                 std::unique_lock<std::mutex> lk(mu);
                 for (;;) {
                     if (stop) return;
-                    // First queued request whose footprint fits (FIFO with backfill).
-                    for (auto it = queue.begin(); it != queue.end() && !r; ++it) {
+                    // Strict FIFO: only the oldest request may start.  Letting later
+                    // requests pass a head that does not fit could starve it forever:
+                    // their outputs stay charged until the caller waits for them, and a
+                    // caller that waits in submission order is blocked on the head.
+                    for (auto it = queue.begin(); it != queue.end() && it == queue.begin() && !r; ++it) {
                         auto& q = *it;
                         auto c = q->key ? cache.find(q->key) : cache.end();
                         const bool can_hit = c != cache.end() && c->second.precision == q->opts.precision &&

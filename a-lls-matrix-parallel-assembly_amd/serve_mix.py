@@ -27,10 +27,12 @@ def _mesh_for(kind, n_elems, seed):
     return alls.generate(alls.GEN_Q4, n, n, jitter=0.2, seed=seed)
 
 
-def build_requests(n_requests, seed=0xA11500C5, min_elems=1e3, max_elems=1e6, reuse=0.6):
+def build_requests(n_requests, seed=0xA11500C5, min_elems=1e3, max_elems=1e6, reuse=0.6, progress=None):
     rng = np.random.default_rng(seed)
     meshes, reqs = [], []
     for i in range(n_requests):
+        if progress and i % 250 == 0:
+            progress(f"generating request {i}/{n_requests}")
         if meshes and rng.random() < reuse:
             mid = int(rng.integers(len(meshes)))
         else:
@@ -51,7 +53,7 @@ def expected_service_ms(m):
     return 0.05 + m.n_elems * per[kind]
 
 
-def run(server, reqs, rate_per_s, seed=1):
+def run(server, reqs, rate_per_s, seed=1, progress=None):
     """Submits reqs at Poisson arrival times; returns per-request stats."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_per_s, size=len(reqs))
@@ -80,8 +82,13 @@ def run(server, reqs, rate_per_s, seed=1):
         ready.acquire()
         if tickets[i] is None:
             continue
-        out, st = server.wait(tickets[i], keep_output=False)
-        stats[i] = st
+        try:
+            out, st = server.wait(tickets[i], keep_output=False)
+            stats[i] = st
+        except alls.AllsError as e:    # a failed request is recorded, the mix goes on
+            errors.append((i, str(e)))
+        if progress and (i + 1) % 200 == 0:
+            progress(f"served {i + 1}/{len(reqs)} requests, {time.perf_counter() - t_start:.1f} s")
     th.join()
     makespan = time.perf_counter() - t_start
     return stats, makespan, errors

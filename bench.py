@@ -200,12 +200,12 @@ def served_mix(n_requests):
     import alls
     import serve_mix
     t0 = time.perf_counter()
-    meshes, reqs = serve_mix.build_requests(n_requests)
+    meshes, reqs = serve_mix.build_requests(n_requests, progress=log)
     t_gen = time.perf_counter() - t0
     busy_s = sum(serve_mix.expected_service_ms(m) for _, m in reqs) / 1e3
     rate = 0.75 * n_requests / busy_s
     with alls.Server(n_streams=4) as srv:
-        stats, makespan, errors = serve_mix.run(srv, reqs, rate)
+        stats, makespan, errors = serve_mix.run(srv, reqs, rate, progress=log)
     res = serve_mix.summarize(reqs, stats, makespan, errors)
     res.update({"arrival_rate_per_s": round(rate, 1), "unique_meshes": len(meshes),
                 "generation_s": round(t_gen, 1), "streams": 4,
@@ -278,6 +278,7 @@ def main():
             extra["hex20"] = hex20_compare(ctx, a.hex_n, 3)
         except Exception as e:  # report, do not hide the main line
             extra["hex20"] = {"error": str(e)}
+    ctx.close()   # frees the context's scratch before the server takes its HBM budget
     if not a.no_extra and a.serve_requests > 0:
         try:
             log(f"served mix: {a.serve_requests} requests")
@@ -286,7 +287,6 @@ def main():
             extra["served_mix"] = {"error": str(e)}
     if extra:
         res["extra"] = extra
-    ctx.close()
     print(json.dumps(res), flush=True)
 
 

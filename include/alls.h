@@ -195,8 +195,9 @@ alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshe
 
 /* ------------------------------------------------------------- serving --- */
 /* Continuous-batching assembly service on one GPU.  Requests are admitted
- * while their HBM footprint fits the budget, run concurrently on up to
- * n_streams streams, and reuse cached patterns when mesh_key repeats. */
+ * in arrival order as soon as a stream is idle and their HBM footprint fits
+ * the budget, run concurrently on up to n_streams streams, and reuse cached
+ * patterns when mesh_key repeats. */
 typedef struct {
     int32_t device;
     int32_t n_streams;        /* concurrent in-flight requests (<= 4 hardware queues by default) */
