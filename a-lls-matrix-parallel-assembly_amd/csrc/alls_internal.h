@@ -174,11 +174,11 @@ void launch_gather_2d(const DevMesh& m, int kind, const Scatter& sc, const int64
 // Hex20 scatter: mode 1 = FP64 atomics, 0 = plain "+=" (one colour),
 // 3 = write each block to its staging slot (value-only path).
 void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int mode, int f32, int32_t* misc, hipStream_t s);
-void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* ci,
+void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
                        const int64_t* inc_off, const int32_t* inc_list, int32_t* cnt, hipStream_t s);
-void launch_slot_map(int64_t n_elems, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
-                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int64_t* slot,
-                     hipStream_t s);
+void launch_slot_map(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
+                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int32_t* fill,
+                     int64_t* slot, hipStream_t s);
 void launch_pair_sum(int64_t n_nodes, const void* rp, int i64, const int64_t* goff, const void* stage, int f32,
                      double* val, hipStream_t s);
 void launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);

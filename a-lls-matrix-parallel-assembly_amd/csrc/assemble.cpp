@@ -10,21 +10,113 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "host_internal.h"
 
 namespace alls {
 
+// ------------------------------------------------ caching device allocator
+// hipFree synchronises the whole device, which would serialise the server's
+// streams.  Freed blocks go to a size-ordered free list and are reused by
+// later requests; they are returned to HIP only when an allocation fails or
+// the cache exceeds a quarter of device memory.  Every free happens after the
+// host has synchronised the work that used the block, so reuse is safe.
+namespace {
+struct Block {
+    size_t bytes;
+    int device;
+};
+std::mutex g_alloc_mu;
+std::multimap<size_t, std::pair<void*, int>> g_free;     // size -> (ptr, device)
+std::unordered_map<void*, Block> g_live;
+size_t g_cached = 0;
+
+size_t round_size(size_t n) {
+    if (n < (size_t(1) << 20)) return (n + 4095) & ~size_t(4095);
+    return (n + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+}
+
+size_t cache_cap() {
+    static size_t cap = 0;
+    if (!cap) {
+        size_t fr = 0, tot = 0;
+        cap = hipMemGetInfo(&fr, &tot) == hipSuccess ? tot / 4 : (size_t(16) << 30);
+    }
+    return cap;
+}
+}  // namespace
+
+void dev_trim() {
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    for (auto& kv : g_free) {
+        (void)hipFree(kv.second.first);
+        g_live.erase(kv.second.first);
+    }
+    g_free.clear();
+    g_cached = 0;
+}
+
+hipError_t dev_alloc(void** out, size_t bytes) {
+    bytes = round_size(std::max<size_t>(bytes, 256));
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    {
+        std::lock_guard<std::mutex> lk(g_alloc_mu);
+        for (auto it = g_free.lower_bound(bytes); it != g_free.end() && it->first <= bytes + bytes / 4; ++it) {
+            if (it->second.second != dev) continue;
+            *out = it->second.first;
+            g_cached -= it->first;
+            g_free.erase(it);
+            return hipSuccess;
+        }
+    }
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        dev_trim();
+        e = hipMalloc(out, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            *out = nullptr;
+            return e;
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_alloc_mu);
+    g_live[*out] = Block{bytes, dev};
+    return hipSuccess;
+}
+
+void dev_free(void* p) {
+    if (!p) return;
+    std::unique_lock<std::mutex> lk(g_alloc_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) {
+        lk.unlock();
+        (void)hipFree(p);
+        return;
+    }
+    if (g_cached + it->second.bytes > cache_cap()) {
+        g_live.erase(it);
+        lk.unlock();
+        (void)hipFree(p);
+        return;
+    }
+    g_cached += it->second.bytes;
+    g_free.emplace(it->second.bytes, std::make_pair(p, it->second.device));
+}
+
 hipError_t DevBuf::ensure(size_t need) {
     if (need == 0) need = 256;
     if (need <= bytes) return hipSuccess;
     release();
     const size_t nb = need + need / 16 + 256;
-    hipError_t e = hipMalloc(&p, nb);
+    hipError_t e = dev_alloc(&p, nb);
     if (e != hipSuccess) {
         p = nullptr;
-        (void)hipGetLastError();
         return e;
     }
     bytes = nb;
@@ -32,7 +124,7 @@ hipError_t DevBuf::ensure(size_t need) {
 }
 
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    dev_free(p);
     p = nullptr;
     bytes = 0;
 }
@@ -201,27 +293,25 @@ alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, De
 
 alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
     if (out->cap_rows < n_rows + 1 || !out->row_ptr) {
-        if (out->row_ptr) (void)hipFree(out->row_ptr);
+        dev_free(out->row_ptr);
         out->row_ptr = nullptr;
         out->cap_rows = 0;
-        if (hipMalloc(&out->row_ptr, (size_t)(n_rows + 1) * sizeof(int64_t)) != hipSuccess) {
+        if (dev_alloc(&out->row_ptr, (size_t)(n_rows + 1) * sizeof(int64_t)) != hipSuccess) {
             out->row_ptr = nullptr;
-            (void)hipGetLastError();
             return ALLS_E_OOM;
         }
         out->cap_rows = n_rows + 1;
         out->pattern_id = 0;
     }
     if (out->cap_nnz < nnz || !out->col_ind || !out->val) {
-        if (out->col_ind) (void)hipFree(out->col_ind);
-        if (out->val) (void)hipFree(out->val);
+        dev_free(out->col_ind);
+        dev_free(out->val);
         out->col_ind = nullptr;
         out->val = nullptr;
         out->cap_nnz = 0;
         const int64_t n = std::max<int64_t>(nnz, 1);
-        if (hipMalloc(reinterpret_cast<void**>(&out->col_ind), (size_t)n * sizeof(int32_t)) != hipSuccess ||
-            hipMalloc(reinterpret_cast<void**>(&out->val), (size_t)n * sizeof(double)) != hipSuccess) {
-            (void)hipGetLastError();
+        if (dev_alloc(reinterpret_cast<void**>(&out->col_ind), (size_t)n * sizeof(int32_t)) != hipSuccess ||
+            dev_alloc(reinterpret_cast<void**>(&out->val), (size_t)n * sizeof(double)) != hipSuccess) {
             return ALLS_E_OOM;
         }
         out->cap_nnz = n;
@@ -235,9 +325,9 @@ alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
 
 void free_csr(alls_csr* out) {
     if (!out) return;
-    if (out->row_ptr) (void)hipFree(out->row_ptr);
-    if (out->col_ind) (void)hipFree(out->col_ind);
-    if (out->val) (void)hipFree(out->val);
+    dev_free(out->row_ptr);
+    dev_free(out->col_ind);
+    dev_free(out->val);
     std::memset(out, 0, sizeof(*out));
 }
 

@@ -8,6 +8,8 @@
 //   k_pair_count : contributions per node pair (elements sharing a and b);
 //   (scan)       : group offsets;
 //   k_slot_map   : slot of each contribution.
+// Both walk each node row's incident elements with one thread, which owns
+// the row's node-pair counters, so they need no atomics.
 // A repeat request then integrates each element (k_hex20 with EMIT 3 writes
 // its 400 blocks to their slots) and k_pair_sum adds every group and writes
 // the 3x3 block into the CSR rows.
@@ -15,55 +17,59 @@
 
 namespace {
 
-// True when node b is one of element e's 20 nodes.
-__device__ __forceinline__ bool has_node(const int32_t* conn, int64_t e, int32_t b) {
+// Local index of node a in element e (a is one of its 20 nodes).
+__device__ __forceinline__ int local_index(const int32_t* conn, int64_t e, int64_t a) {
     const int32_t* c = conn + e * 20;
-    bool f = false;
+    int I = 0;
 #pragma unroll
-    for (int i = 0; i < 20; ++i) f |= c[i] == b;
-    return f;
+    for (int i = 0; i < 20; ++i)
+        if (c[i] == a) I = i;
+    return I;
 }
 
-// Thread per node row a: for every rank r, the number of elements of a's
-// incidence list that also hold column node b_r.
+// Thread per node row a: the row owns its node pairs g0 .. g0+u-1, so the
+// counters need no atomics.  For every element incident to a, every one of
+// its 20 column nodes adds one contribution to pair (a, rank).
 template <typename RP>
 __global__ __launch_bounds__(256) void k_pair_count(int64_t n_nodes, const int32_t* conn, const RP* rp,
-                                                    const int32_t* ci, const int64_t* inc_off,
+                                                    const int32_t* rank_map, const int64_t* inc_off,
                                                     const int32_t* inc_list, int32_t* cnt) {
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n_nodes) return;
     const int64_t r0 = (int64_t)rp[3 * a];
     const int64_t u = ((int64_t)rp[3 * a + 1] - r0) / 3;
     const int64_t g0 = r0 / 9;
-    for (int64_t r = 0; r < u; ++r) {
-        const int32_t b = ci[r0 + 3 * r] / 3;
-        int32_t n = 0;
-        for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) n += has_node(conn, inc_list[k], b) ? 1 : 0;
-        cnt[g0 + r] = n;
+    for (int64_t r = 0; r < u; ++r) cnt[g0 + r] = 0;
+    for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) {
+        const int64_t e = inc_list[k];
+        const int32_t* mp = rank_map + e * 400 + local_index(conn, e, a) * 20;
+        for (int J = 0; J < 20; ++J) cnt[g0 + mp[J]] += 1;
     }
 }
 
-// Thread per (element, row node I): slot of each of the 20 contributions.
+// Thread per node row a, elements in ascending id (the incidence list is
+// sorted): contribution (e, I, J) takes the next free slot of its pair's
+// group, so groups come out in the serial order of Acc:336.  `fill` holds
+// the row's running counts (zeroed here).
 template <typename RP>
-__global__ __launch_bounds__(256) void k_slot_map(int64_t n_elems, const int32_t* conn, const RP* rp,
+__global__ __launch_bounds__(256) void k_slot_map(int64_t n_nodes, const int32_t* conn, const RP* rp,
                                                   const int32_t* rank_map, const int64_t* inc_off,
-                                                  const int32_t* inc_list, const int64_t* goff, int64_t* slot) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t e = t / 20;
-    const int I = (int)(t % 20);
-    if (e >= n_elems) return;
-    const int32_t* c = conn + e * 20;
-    const int64_t a = c[I];
-    const int64_t lo = inc_off[a];
-    int64_t k = lo;
-    while (k < inc_off[a + 1] && inc_list[k] != e) ++k;
-    if (k == inc_off[a + 1]) k = lo;   // not reachable for a valid incidence; keeps slots in bounds
-    const int64_t g0 = (int64_t)rp[3 * a] / 9;
-    for (int J = 0; J < 20; ++J) {
-        const int32_t b = c[J];
-        int64_t pos = 0;
-        for (int64_t q = lo; q < k; ++q) pos += has_node(conn, inc_list[q], b) ? 1 : 0;
-        slot[e * 400 + I * 20 + J] = goff[g0 + rank_map[e * 400 + I * 20 + J]] + pos;
+                                                  const int32_t* inc_list, const int64_t* goff, int32_t* fill,
+                                                  int64_t* slot) {
+    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n_nodes) return;
+    const int64_t r0 = (int64_t)rp[3 * a];
+    const int64_t u = ((int64_t)rp[3 * a + 1] - r0) / 3;
+    const int64_t g0 = r0 / 9;
+    for (int64_t r = 0; r < u; ++r) fill[g0 + r] = 0;
+    for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) {
+        const int64_t e = inc_list[k];
+        const int I = local_index(conn, e, a);
+        const int32_t* mp = rank_map + e * 400 + I * 20;
+        for (int J = 0; J < 20; ++J) {
+            const int64_t g = g0 + mp[J];
+            slot[e * 400 + I * 20 + J] = goff[g] + fill[g]++;
+        }
     }
 }
 
@@ -102,27 +108,27 @@ unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256 > 0 ? (n + 255)
 
 namespace alls {
 
-void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* ci,
+void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
                        const int64_t* inc_off, const int32_t* inc_list, int32_t* cnt, hipStream_t s) {
     if (n_nodes <= 0) return;
     if (i64)
         hipLaunchKernelGGL(k_pair_count<int64_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
-                           static_cast<const int64_t*>(rp), ci, inc_off, inc_list, cnt);
+                           static_cast<const int64_t*>(rp), rank_map, inc_off, inc_list, cnt);
     else
         hipLaunchKernelGGL(k_pair_count<int32_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
-                           static_cast<const int32_t*>(rp), ci, inc_off, inc_list, cnt);
+                           static_cast<const int32_t*>(rp), rank_map, inc_off, inc_list, cnt);
 }
 
-void launch_slot_map(int64_t n_elems, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
-                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int64_t* slot,
-                     hipStream_t s) {
-    if (n_elems <= 0) return;
+void launch_slot_map(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
+                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int32_t* fill,
+                     int64_t* slot, hipStream_t s) {
+    if (n_nodes <= 0) return;
     if (i64)
-        hipLaunchKernelGGL(k_slot_map<int64_t>, dim3(grid_for(n_elems * 20)), dim3(256), 0, s, n_elems, conn,
-                           static_cast<const int64_t*>(rp), rank_map, inc_off, inc_list, goff, slot);
+        hipLaunchKernelGGL(k_slot_map<int64_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
+                           static_cast<const int64_t*>(rp), rank_map, inc_off, inc_list, goff, fill, slot);
     else
-        hipLaunchKernelGGL(k_slot_map<int32_t>, dim3(grid_for(n_elems * 20)), dim3(256), 0, s, n_elems, conn,
-                           static_cast<const int32_t*>(rp), rank_map, inc_off, inc_list, goff, slot);
+        hipLaunchKernelGGL(k_slot_map<int32_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
+                           static_cast<const int32_t*>(rp), rank_map, inc_off, inc_list, goff, fill, slot);
 }
 
 void launch_pair_sum(int64_t n_nodes, const void* rp, int i64, const int64_t* goff, const void* stage, int f32,

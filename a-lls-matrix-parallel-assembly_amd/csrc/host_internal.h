@@ -10,6 +10,11 @@
 
 namespace alls {
 
+// Caching device allocator (free lists; no device-wide sync on free).
+hipError_t dev_alloc(void** out, size_t bytes);
+void dev_free(void* p);
+void dev_trim();
+
 // Grow-only device buffer (owns its allocation; movable, not copyable).
 struct DevBuf {
     void* p = nullptr;

@@ -121,16 +121,26 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
         for (int i = 0; i < 4; ++i) {
             if (i >= el.nn || el.nd[i] != a) continue;
             const int32_t* mp = sc.map + e * 16 + i * 4;
+            // All loads of the element's blocks first, then all stores: the
+            // element's columns are distinct, and issuing the loads together
+            // keeps them in flight instead of chaining load-add-store.
+            double* v[4];
+            double old[4][4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j >= el.nn) continue;
+                v[j] = sc.val + r0 + 2 * (int64_t)mp[j];
+                const double2 lo = *reinterpret_cast<const double2*>(v[j]);
+                const double2 hi = *reinterpret_cast<const double2*>(v[j] + rowlen);
+                old[j][0] = lo.x; old[j][1] = lo.y; old[j][2] = hi.x; old[j][3] = hi.y;
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (j >= el.nn) continue;
                 double kb[4];
                 el.block(i, j, kb);
-                double* v = sc.val + r0 + 2 * (int64_t)mp[j];
-                v[0] += kb[0];
-                v[1] += kb[1];
-                v[rowlen] += kb[2];
-                v[rowlen + 1] += kb[3];
+                *reinterpret_cast<double2*>(v[j]) = make_double2(old[j][0] + kb[0], old[j][1] + kb[1]);
+                *reinterpret_cast<double2*>(v[j] + rowlen) = make_double2(old[j][2] + kb[2], old[j][3] + kb[3]);
             }
         }
     }

@@ -172,12 +172,14 @@ alls_status build_hex20_slots(alls_pattern* p, hipStream_t s, std::string* err) 
     HIP_TRY(p->slot.ensure(std::max<int64_t>(p->dm.n_elems, 1) * 400 * sizeof(int64_t)), "alloc slot map");
     DevBuf tiles;
     HIP_TRY(tiles.ensure((scan_tile_count(npairs) + 1) * sizeof(int64_t)), "alloc scan tiles");
-    launch_pair_count(p->dm.n_nodes, p->dm.conn, p->row_ptr.p, p->i64, p->col_ind.as<int32_t>(),
+    launch_pair_count(p->dm.n_nodes, p->dm.conn, p->row_ptr.p, p->i64, p->map.as<int32_t>(),
                       p->inc_off.as<int64_t>(), p->inc_list.as<int32_t>(), p->pair_cnt.as<int32_t>(), s);
     launch_scan_total(p->pair_cnt.as<int32_t>(), npairs, tiles.as<int64_t>(), scan_tile_count(npairs), s);
     launch_scan_apply(p->pair_cnt.as<int32_t>(), npairs, tiles.as<int64_t>(), 1, p->goff.p, 1, s);
-    launch_slot_map(p->dm.n_elems, p->dm.conn, p->row_ptr.p, p->i64, p->map.as<int32_t>(), p->inc_off.as<int64_t>(),
-                    p->inc_list.as<int32_t>(), p->goff.as<int64_t>(), p->slot.as<int64_t>(), s);
+    // pair_cnt is reused as the running fill counters of the slot map.
+    launch_slot_map(p->dm.n_nodes, p->dm.conn, p->row_ptr.p, p->i64, p->map.as<int32_t>(), p->inc_off.as<int64_t>(),
+                    p->inc_list.as<int32_t>(), p->goff.as<int64_t>(), p->pair_cnt.as<int32_t>(),
+                    p->slot.as<int64_t>(), s);
     HIP_TRY(hipGetLastError(), "launch slot map");
     HIP_TRY(hipMemcpyAsync(&p->n_contrib, p->goff.as<int64_t>() + npairs, sizeof(int64_t), hipMemcpyDeviceToHost, s),
             "read contributions");

@@ -218,7 +218,8 @@ struct alls_server {
                 if (rc == ALLS_OK) rc = stage_mesh(w, &r->mesh, mi, &dm, s, &r->err);
                 if (rc == ALLS_OK) rc = run_alls(w, dm, mi, r->opts, &r->out, s, &st, &r->err);
             }
-            // Scratch above 1 GiB is returned so idle slots do not pin HBM.
+            // Scratch above 1 GiB goes back to the allocator cache so an idle
+            // slot does not pin HBM that the next admitted request may need.
             for (DevBuf* b : {&w.keys, &w.vals})
                 if (b->bytes > (size_t(1) << 30)) b->release();
             {
@@ -251,7 +252,7 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
     s->cfg = *cfg;
     const int n = std::min(std::max(cfg->n_streams, 1), 16);
     s->cfg.n_streams = n;
-    if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 64;
+    if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 1 << 20;   // bounded by the budget
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
         delete s;
