@@ -23,7 +23,8 @@ namespace alls {
 // hipFree synchronises the whole device, which would serialise the server's
 // streams.  Freed blocks go to a size-ordered free list and are reused by
 // later requests; they are returned to HIP only when an allocation fails or
-// the cache exceeds a quarter of device memory.  Every free happens after the
+// the cache exceeds a quarter of device memory (the server's default budget
+// leaves that much HBM free).  Every free happens after the
 // host has synchronised the work that used the block, so reuse is safe.
 namespace {
 struct Block {
@@ -48,16 +49,25 @@ size_t cache_cap() {
     }
     return cap;
 }
+
+// Frees cached blocks, largest first, until `need` bytes are returned (all
+// of them when need == 0).  Caller holds g_alloc_mu.
+void trim_locked(size_t need) {
+    size_t freed = 0;
+    while (!g_free.empty() && (need == 0 || freed < need)) {
+        auto it = std::prev(g_free.end());
+        (void)hipFree(it->second.first);
+        g_live.erase(it->second.first);
+        freed += it->first;
+        g_cached -= it->first;
+        g_free.erase(it);
+    }
+}
 }  // namespace
 
 void dev_trim() {
     std::lock_guard<std::mutex> lk(g_alloc_mu);
-    for (auto& kv : g_free) {
-        (void)hipFree(kv.second.first);
-        g_live.erase(kv.second.first);
-    }
-    g_free.clear();
-    g_cached = 0;
+    trim_locked(0);
 }
 
 hipError_t dev_alloc(void** out, size_t bytes) {
@@ -65,8 +75,9 @@ hipError_t dev_alloc(void** out, size_t bytes) {
     int dev = 0;
     (void)hipGetDevice(&dev);
     {
+        // Smallest cached block that holds the request and wastes at most half.
         std::lock_guard<std::mutex> lk(g_alloc_mu);
-        for (auto it = g_free.lower_bound(bytes); it != g_free.end() && it->first <= bytes + bytes / 4; ++it) {
+        for (auto it = g_free.lower_bound(bytes); it != g_free.end() && it->first <= 2 * bytes; ++it) {
             if (it->second.second != dev) continue;
             *out = it->second.first;
             g_cached -= it->first;
@@ -77,8 +88,16 @@ hipError_t dev_alloc(void** out, size_t bytes) {
     hipError_t e = hipMalloc(out, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        dev_trim();
+        {
+            std::lock_guard<std::mutex> lk(g_alloc_mu);
+            trim_locked(2 * bytes);
+        }
         e = hipMalloc(out, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            dev_trim();
+            e = hipMalloc(out, bytes);
+        }
         if (e != hipSuccess) {
             (void)hipGetLastError();
             *out = nullptr;
@@ -99,12 +118,7 @@ void dev_free(void* p) {
         (void)hipFree(p);
         return;
     }
-    if (g_cached + it->second.bytes > cache_cap()) {
-        g_live.erase(it);
-        lk.unlock();
-        (void)hipFree(p);
-        return;
-    }
+    if (g_cached + it->second.bytes > cache_cap()) trim_locked(it->second.bytes);
     g_cached += it->second.bytes;
     g_free.emplace(it->second.bytes, std::make_pair(p, it->second.device));
 }
@@ -147,10 +161,44 @@ hipError_t Workspace::init() {
     return hipSuccess;
 }
 
+hipError_t Workspace::upload(void* const* dst, const void* const* src, const size_t* bytes, int n, hipStream_t s) {
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) total += (bytes[i] + 255) & ~size_t(255);
+    if (total == 0) return hipSuccess;
+    // The previous upload from this buffer has completed: every pipeline
+    // synchronises its stream before returning.
+    if (total > pinned_bytes) {
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
+        pinned_bytes = 0;
+        const size_t nb = total + total / 4;
+        hipError_t e = hipHostMalloc(&pinned, nb, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            pinned = nullptr;
+            (void)hipGetLastError();
+            return e;
+        }
+        pinned_bytes = nb;
+    }
+    size_t off = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!bytes[i]) continue;
+        char* stage = static_cast<char*>(pinned) + off;
+        std::memcpy(stage, src[i], bytes[i]);
+        hipError_t e = hipMemcpyAsync(dst[i], stage, bytes[i], hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        off += (bytes[i] + 255) & ~size_t(255);
+    }
+    return hipSuccess;
+}
+
 void Workspace::destroy() {
     for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b->release();
     if (host_misc) (void)hipHostFree(host_misc);
     host_misc = nullptr;
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_bytes = 0;
     if (events)
         for (auto& x : ev) (void)hipEventDestroy(x);
     events = false;
@@ -251,40 +299,40 @@ alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, De
     d.thickness = m->thickness;
     const size_t cb = (size_t)m->n_nodes * m->dim * sizeof(double);
     const size_t kb = (size_t)m->n_elems * m->nodes_per_elem * sizeof(int32_t);
-    const bool host = m->mem == ALLS_MEM_HOST;
-    const hipMemcpyKind kind = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-    // Device inputs are used in place when aligned for vector loads.
-    if (!host && aligned16(m->coords)) {
-        d.coords = m->coords;
-    } else if (cb) {
+    const size_t tb = m->elem_type ? (size_t)m->n_elems : 0;
+    const size_t eb = m->E_per_elem ? (size_t)m->n_elems * sizeof(double) : 0;
+    if (m->mem == ALLS_MEM_HOST) {
+        // Host mesh: one pinned staging pass, four async copies.
         HIP_TRY(ws.coords.ensure(cb), "alloc coords");
-        HIP_TRY(hipMemcpyAsync(ws.coords.p, m->coords, cb, kind, s), "copy coords");
-        d.coords = ws.coords.as<double>();
-    }
-    if (!host && aligned16(m->conn)) {
-        d.conn = m->conn;
-    } else if (kb) {
         HIP_TRY(ws.conn.ensure(kb), "alloc conn");
-        HIP_TRY(hipMemcpyAsync(ws.conn.p, m->conn, kb, kind, s), "copy conn");
-        d.conn = ws.conn.as<int32_t>();
-    }
-    if (m->elem_type) {
-        if (!host) {
-            d.types = m->elem_type;
-        } else if (m->n_elems) {
-            HIP_TRY(ws.types.ensure((size_t)m->n_elems), "alloc types");
-            HIP_TRY(hipMemcpyAsync(ws.types.p, m->elem_type, (size_t)m->n_elems, kind, s), "copy types");
-            d.types = ws.types.as<uint8_t>();
+        if (tb) HIP_TRY(ws.types.ensure(tb), "alloc types");
+        if (eb) HIP_TRY(ws.E.ensure(eb), "alloc E");
+        void* dst[4] = {ws.coords.p, ws.conn.p, ws.types.p, ws.E.p};
+        const void* src[4] = {m->coords, m->conn, m->elem_type, m->E_per_elem};
+        const size_t bytes[4] = {cb, kb, tb, eb};
+        HIP_TRY(ws.upload(dst, src, bytes, 4, s), "upload mesh");
+        d.coords = cb ? ws.coords.as<double>() : nullptr;
+        d.conn = kb ? ws.conn.as<int32_t>() : nullptr;
+        d.types = tb ? ws.types.as<uint8_t>() : nullptr;
+        d.E_elem = eb ? ws.E.as<double>() : nullptr;
+    } else {
+        // Device inputs are used in place when aligned for vector loads.
+        if (aligned16(m->coords) || !cb) {
+            d.coords = m->coords;
+        } else {
+            HIP_TRY(ws.coords.ensure(cb), "alloc coords");
+            HIP_TRY(hipMemcpyAsync(ws.coords.p, m->coords, cb, hipMemcpyDeviceToDevice, s), "copy coords");
+            d.coords = ws.coords.as<double>();
         }
-    }
-    if (m->E_per_elem) {
-        if (!host) {
-            d.E_elem = m->E_per_elem;
-        } else if (m->n_elems) {
-            HIP_TRY(ws.E.ensure((size_t)m->n_elems * sizeof(double)), "alloc E");
-            HIP_TRY(hipMemcpyAsync(ws.E.p, m->E_per_elem, (size_t)m->n_elems * sizeof(double), kind, s), "copy E");
-            d.E_elem = ws.E.as<double>();
+        if (aligned16(m->conn) || !kb) {
+            d.conn = m->conn;
+        } else {
+            HIP_TRY(ws.conn.ensure(kb), "alloc conn");
+            HIP_TRY(hipMemcpyAsync(ws.conn.p, m->conn, kb, hipMemcpyDeviceToDevice, s), "copy conn");
+            d.conn = ws.conn.as<int32_t>();
         }
+        d.types = m->elem_type;
+        d.E_elem = m->E_per_elem;
     }
     (void)mi;
     *dm = d;
@@ -363,10 +411,16 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     const int G = small_group(mi.kind);
     const int64_t ntiles = scan_tile_count(n_nodes);
     int retries = 0;
+    bool full_pool = false;
     ListPool lp{};
     int64_t total = 0;
     for (;;) {
-        const int64_t cap_chunks = n_nodes + (mi.entries_bound + C - 1) / C + 1;
+        // Overflow chunks: the exact bound (entries / C) is rarely approached,
+        // since most rows fit their home chunk; reserve a quarter of it plus a
+        // chunk per 4 rows and retry with the full bound if the pool runs out.
+        const int64_t bound = (mi.entries_bound + C - 1) / C + 1;
+        const int64_t ovf = full_pool ? bound : std::min(bound, bound / 4 + n_nodes / 4 + 1024);
+        const int64_t cap_chunks = n_nodes + ovf;
         HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
         HIP_TRY(ws.keys.ensure((size_t)cap_chunks * C * sizeof(int2)), "alloc list keys");
         HIP_TRY(ws.vals.ensure((size_t)cap_chunks * C * B * vsz), "alloc list values");
@@ -417,9 +471,13 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
                 if (err) *err = "list directory overflow after retries";
                 return ALLS_E_POOL_OVERFLOW;
             }
-            int nc = 2 * C;
-            while ((int64_t)nc * ALLS_DIR < need) nc *= 2;
-            C = nc;
+            if (need > 0 || f[MISC_ERR + ERR_ROW_TOO_BIG]) {
+                int nc = 2 * C;
+                while ((int64_t)nc * ALLS_DIR < need) nc *= 2;
+                C = nc;
+            } else {
+                full_pool = true;
+            }
             continue;
         }
         total = ws.host_misc[32];

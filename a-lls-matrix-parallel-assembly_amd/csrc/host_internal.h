@@ -54,6 +54,11 @@ struct Workspace {
     DevBuf hdr, keys, vals, misc, u, big, tiles;
     DevBuf coords, conn, types, E;
     int64_t* host_misc = nullptr;   // pinned readback: misc ints + scan total
+    void* pinned = nullptr;         // pinned staging of host inputs (grow-only)
+    size_t pinned_bytes = 0;
+    // Copies host arrays to device through the pinned buffer (one async copy
+    // each); pageable copies would block behind other streams' work.
+    hipError_t upload(void* const* dst, const void* const* src, const size_t* bytes, int n, hipStream_t s);
     hipEvent_t ev[8] = {};
     bool events = false;
     hipError_t init();

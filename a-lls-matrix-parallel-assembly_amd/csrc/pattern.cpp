@@ -122,9 +122,11 @@ alls_status stage_E(alls_ctx* ctx, const alls_pattern* p, const double* E, int32
         *out = E;
         return ALLS_OK;
     }
-    HIP_TRY(ctx->ws.E.ensure(std::max<int64_t>(p->dm.n_elems, 1) * sizeof(double)), "alloc E");
-    HIP_TRY(hipMemcpyAsync(ctx->ws.E.p, E, p->dm.n_elems * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
-            "copy E");
+    const size_t eb = (size_t)p->dm.n_elems * sizeof(double);
+    HIP_TRY(ctx->ws.E.ensure(std::max<size_t>(eb, 8)), "alloc E");
+    void* dst[1] = {ctx->ws.E.p};
+    const void* src[1] = {E};
+    HIP_TRY(ctx->ws.upload(dst, src, &eb, 1, ctx->stream), "upload E");
     *out = ctx->ws.E.as<double>();
     return ALLS_OK;
 }

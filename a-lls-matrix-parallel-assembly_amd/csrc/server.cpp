@@ -43,6 +43,7 @@ struct Request {
     alls_status status = ALLS_OK;
     bool done = false;
     bool cache_hit = false;
+    bool built = false;
     alls_csr out{};
     std::string err;
 };
@@ -178,9 +179,11 @@ struct alls_server {
                 const double* E = r->mesh.E_per_elem;
                 rc = ALLS_OK;
                 if (E && r->mesh.mem == ALLS_MEM_HOST) {
-                    hipError_t e = w.E.ensure((size_t)std::max<int64_t>(r->mesh.n_elems, 1) * sizeof(double));
-                    if (e == hipSuccess)
-                        e = hipMemcpyAsync(w.E.p, E, (size_t)r->mesh.n_elems * sizeof(double), hipMemcpyHostToDevice, s);
+                    const size_t eb = (size_t)r->mesh.n_elems * sizeof(double);
+                    hipError_t e = w.E.ensure(std::max<size_t>(eb, 8));
+                    void* dst[1] = {w.E.p};
+                    const void* src[1] = {E};
+                    if (e == hipSuccess) e = w.upload(dst, src, &eb, 1, s);
                     if (e != hipSuccess) rc = hip_fail(e, "stage E", &r->err);
                     E = w.E.as<double>();
                 }
@@ -227,6 +230,7 @@ struct alls_server {
                 r->t_done = Clock::now();
                 r->status = rc;
                 r->cache_hit = hit;
+                r->built = build && rc == ALLS_OK;
                 r->done = true;
                 if (hit) ce->refs--;
                 // The output stays charged until the caller releases it.
@@ -258,7 +262,9 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
         delete s;
         return ALLS_E_HIP;
     }
-    s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)(0.8 * (double)free_b);
+    // Default budget: 55% of free HBM; the allocator may keep up to a quarter
+    // of HBM in freed blocks for reuse, so live + cached stays below capacity.
+    s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)(0.55 * (double)free_b);
     for (int i = 0; i < n; ++i) {
         hipStream_t st;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
@@ -319,7 +325,7 @@ alls_status alls_server_wait(alls_server* s, int64_t ticket, alls_csr* out, alls
         st->run_ms = ms_between(r->t_admit, r->t_done);
         st->latency_ms = ms_between(r->t_submit, r->t_done);
         st->nnz = r->out.nnz;
-        st->cache_hit = r->cache_hit ? 1 : 0;
+        st->cache_hit = r->cache_hit ? 1 : r->built ? 2 : 0;
         st->status = r->status;
     }
     if (out) {

@@ -111,10 +111,22 @@ def run(server, reqs, rate_per_s, seed=1, progress=None):
 def summarize(reqs, stats, makespan, errors):
     lat = np.array([s["latency_ms"] for s in stats if s is not None])
     q = np.array([s["queue_ms"] for s in stats if s is not None])
-    hits = sum(1 for s in stats if s is not None and s["cache_hit"])
+    hits = sum(1 for s in stats if s is not None and s["cache_hit"] == 1)
+    paths = {}
+    for (_, m), st in zip(reqs, stats):
+        if st is None:
+            continue
+        k = ("hex20" if m.dim == 3 else "2d") + "_" + ("hit", "plain", "build")[[1, 0, 2].index(st["cache_hit"])]
+        d = paths.setdefault(k, {"n": 0, "elements": 0, "run_ms": 0.0})
+        d["n"] += 1
+        d["elements"] += m.n_elems
+        d["run_ms"] += st["run_ms"]
+    for d in paths.values():
+        d["ns_per_element"] = round(d["run_ms"] * 1e6 / max(d["elements"], 1), 2)
+        d["run_ms"] = round(d["run_ms"], 1)
     elems = sum(m.n_elems for (_, m), s in zip(reqs, stats) if s is not None)
     return {"requests": len(reqs), "completed": int(lat.size), "refused": len(errors),
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_ms": round(float(lat.mean()), 3), "queue_p99_ms": round(float(np.percentile(q, 99)), 3),
-            "cache_hits": hits, "elements": int(elems), "makespan_s": round(makespan, 3),
+            "cache_hits": hits, "paths": paths, "elements": int(elems), "makespan_s": round(makespan, 3),
             "elements_per_s": round(elems / makespan, 1)}

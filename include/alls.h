@@ -201,7 +201,7 @@ alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshe
 typedef struct {
     int32_t device;
     int32_t n_streams;        /* concurrent in-flight requests (<= 4 hardware queues by default) */
-    int64_t hbm_budget_bytes; /* admission budget; 0 => 80% of free HBM */
+    int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM */
     int32_t pattern_cache_entries;
     int32_t reserved;
 } alls_server_config;
@@ -211,7 +211,8 @@ typedef struct {
     double run_ms;            /* admitted -> done (device work of this request) */
     double latency_ms;        /* submit -> done */
     int64_t nnz;
-    int32_t cache_hit;        /* 1 => value-only assembly on a cached pattern */
+    int32_t cache_hit;        /* 1 => value-only assembly on a cached pattern, 2 => this request
+                                 built and cached the pattern, 0 => plain assembly */
     int32_t status;           /* alls_status of the request */
 } alls_request_stats;
 
