@@ -192,6 +192,16 @@ hipError_t Workspace::upload(void* const* dst, const void* const* src, const siz
     return hipSuccess;
 }
 
+size_t Workspace::device_bytes() const {
+    size_t b = 0;
+    for (const DevBuf* x : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b += x->bytes;
+    return b;
+}
+
+void Workspace::release_device() {
+    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b->release();
+}
+
 void Workspace::destroy() {
     for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b->release();
     if (host_misc) (void)hipHostFree(host_misc);

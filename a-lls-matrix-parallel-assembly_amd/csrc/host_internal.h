@@ -63,6 +63,8 @@ struct Workspace {
     bool events = false;
     hipError_t init();
     void destroy();
+    size_t device_bytes() const;
+    void release_device();          // frees every device buffer, keeps events and pinned memory
 };
 
 alls_status classify_mesh(const alls_mesh* m, const alls_opts& o, MeshInfo* mi, std::string* err);

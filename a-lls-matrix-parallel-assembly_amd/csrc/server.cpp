@@ -37,8 +37,9 @@ struct Request {
     alls_mesh mesh{};
     uint64_t key = 0;
     alls_opts opts{};
-    int64_t footprint = 0;
-    int64_t pattern_bytes = 0;      // extra bytes when this request builds a cached pattern
+    int64_t footprint = 0;          // charged while running (output + pattern, see estimate_footprint)
+    int64_t pattern_bytes = 0;      // part of footprint when this request builds a cached pattern
+    int64_t scratch = 0;            // list pool + scratch, held by the worker's workspace
     Clock::time_point t_submit, t_admit, t_done;
     alls_status status = ALLS_OK;
     bool done = false;
@@ -57,13 +58,16 @@ struct CacheEntry {
     int64_t n_elems = 0, n_nodes = 0;
 };
 
-// Upper-bound device bytes of one assembly (list pool + scratch + output).
-int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pattern, int64_t* pattern_bytes) {
+// Upper-bound device bytes of one assembly: returns output (+ pattern) bytes;
+// *scratch gets the list pool and workspace bytes.
+int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pattern, int64_t* pattern_bytes,
+                           int64_t* scratch_out) {
     const int64_t B = mi.dpn * mi.dpn;
     const int64_t vsz = mi.f32 ? 4 : 8;
     const int64_t ent = mi.entries_bound;
     const int64_t C = mi.kind == 3 ? 80 : mi.kind == 0 ? 16 : 24;
-    const int64_t chunks = m->n_nodes + ent / C + 1;
+    const int64_t bound = ent / C + 1;
+    const int64_t chunks = m->n_nodes + std::min(bound, bound / 4 + m->n_nodes / 4 + 1024);
     const int64_t n_rows = m->n_nodes * mi.dpn;
     const int64_t nnz_bound = ent * B;
     int64_t scratch = chunks * C * (8 + B * vsz) + m->n_nodes * (32 + 8) + 1024;
@@ -77,7 +81,8 @@ int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pat
         if (mi.kind == 3) pat += (nnz_bound / 9 + 1) * 12 + m->n_elems * 400 * 8;
     }
     if (pattern_bytes) *pattern_bytes = pat;
-    return scratch + inputs + out + pat;
+    if (scratch_out) *scratch_out = scratch + inputs;
+    return out + pat;
 }
 
 }  // namespace
@@ -90,7 +95,9 @@ struct alls_server {
     std::unordered_map<int64_t, std::shared_ptr<Request>> reqs;
     std::unordered_map<uint64_t, CacheEntry> cache;
     std::unordered_map<uint64_t, int> building;   // keys whose pattern is being built
-    int64_t budget = 0, in_use = 0;
+    int64_t budget = 0, in_use = 0;   // in_use: running requests + unreleased outputs
+    int64_t ws_total = 0;             // bytes held by the workers' grow-only workspaces
+    std::vector<int64_t> ws_bytes;    // per worker, updated by that worker under the lock
     int64_t next_ticket = 1;
     uint64_t use_clock = 0;
     bool stop = false;
@@ -105,7 +112,7 @@ struct alls_server {
     }
     // Frees least-recently used idle patterns until `need` bytes fit.
     void evict_for(int64_t need) {
-        while (in_use + cache_bytes() + need > budget) {
+        while (in_use + ws_total + cache_bytes() + need > budget) {
             auto victim = cache.end();
             for (auto it = cache.begin(); it != cache.end(); ++it)
                 if (it->second.refs == 0 && (victim == cache.end() || it->second.last_use < victim->second.last_use))
@@ -132,20 +139,24 @@ struct alls_server {
                     // requests pass a head that does not fit could starve it forever:
                     // their outputs stay charged until the caller waits for them, and a
                     // caller that waits in submission order is blocked on the head.
-                    for (auto it = queue.begin(); it != queue.end() && it == queue.begin() && !r; ++it) {
-                        auto& q = *it;
+                    // Two tries: the second after this idle worker gives its
+                    // workspace back, so held scratch can never starve the head.
+                    for (int attempt = 0; attempt < 2 && !r && !queue.empty(); ++attempt) {
+                        std::shared_ptr<Request> q = queue.front();
                         auto c = q->key ? cache.find(q->key) : cache.end();
                         const bool can_hit = c != cache.end() && c->second.precision == q->opts.precision &&
                                              c->second.n_elems == q->mesh.n_elems &&
                                              c->second.n_nodes == q->mesh.n_nodes;
                         const bool can_build = q->key && !can_hit && !building.count(q->key) &&
                                                (c == cache.end() || c->second.refs == 0);
-                        const int64_t need = can_build ? q->footprint : q->footprint - q->pattern_bytes;
+                        // This worker's workspace is reused; only its growth is new memory.
+                        const int64_t grow = std::max<int64_t>(0, q->scratch - (int64_t)w.device_bytes());
+                        const int64_t need = (can_build ? q->footprint : q->footprint - q->pattern_bytes) + grow;
                         if (can_hit) c->second.refs++;   // pinned while we make room
                         int64_t evictable = 0;
                         for (auto& kv : cache)
                             if (kv.second.refs == 0) evictable += kv.second.bytes;
-                        if (in_use + cache_bytes() - evictable + need <= budget) {
+                        if (in_use + ws_total + cache_bytes() - evictable + need <= budget) {
                             evict_for(need);
                             auto stale = can_build ? cache.find(q->key) : cache.end();
                             if (stale != cache.end()) {   // idle entry of another mesh under this key
@@ -153,7 +164,7 @@ struct alls_server {
                                 cache.erase(stale);
                             }
                             r = q;
-                            queue.erase(it);
+                            queue.pop_front();
                             hit = can_hit;
                             build = can_build;
                             if (hit) {
@@ -161,11 +172,17 @@ struct alls_server {
                                 ce->last_use = ++use_clock;
                             }
                             if (build) building[r->key] = 1;
-                            r->footprint = need;
-                            in_use += need;
+                            r->footprint = need - grow;
+                            in_use += need - grow;
+                            ws_total += grow;   // trued up to the real size when the request ends
                             break;
                         }
                         if (can_hit) c->second.refs--;
+                        if (w.device_bytes() == 0) break;
+                        w.release_device();
+                        ws_bytes[slot] = 0;
+                        ws_total = 0;
+                        for (int64_t b : ws_bytes) ws_total += b;
                     }
                     if (r) break;
                     cv_work.wait(lk);
@@ -221,12 +238,11 @@ struct alls_server {
                 if (rc == ALLS_OK) rc = stage_mesh(w, &r->mesh, mi, &dm, s, &r->err);
                 if (rc == ALLS_OK) rc = run_alls(w, dm, mi, r->opts, &r->out, s, &st, &r->err);
             }
-            // Scratch above 1 GiB goes back to the allocator cache so an idle
-            // slot does not pin HBM that the next admitted request may need.
-            for (DevBuf* b : {&w.keys, &w.vals})
-                if (b->bytes > (size_t(1) << 30)) b->release();
             {
                 std::lock_guard<std::mutex> lk(mu);
+                ws_bytes[slot] = (int64_t)w.device_bytes();
+                ws_total = 0;
+                for (int64_t b : ws_bytes) ws_total += b;
                 r->t_done = Clock::now();
                 r->status = rc;
                 r->cache_hit = hit;
@@ -278,6 +294,7 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
             return ALLS_E_HIP;
         }
     }
+    s->ws_bytes.assign(n, 0);
     for (int i = 0; i < n; ++i) s->workers.emplace_back(&alls_server::worker, s, i);
     *out = s;
     return ALLS_OK;
@@ -294,9 +311,9 @@ alls_status alls_server_submit(alls_server* s, const alls_mesh* mesh, uint64_t k
     std::string err;
     alls_status rc = classify_mesh(mesh, r->opts, &mi, &err);
     if (rc != ALLS_OK) return rc;
-    r->footprint = estimate_footprint(mesh, mi, key != 0, &r->pattern_bytes);
-    if (r->footprint - r->pattern_bytes > s->budget) return ALLS_E_BUSY;
-    if (r->footprint > s->budget) r->key = 0;   // fits only without caching its pattern
+    r->footprint = estimate_footprint(mesh, mi, key != 0, &r->pattern_bytes, &r->scratch);
+    if (r->footprint - r->pattern_bytes + r->scratch > s->budget) return ALLS_E_BUSY;
+    if (r->footprint + r->scratch > s->budget) r->key = 0;   // fits only without caching its pattern
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);
