@@ -31,6 +31,7 @@ enum {
     MISC_ERR = 0,              // ERR_SLOTS entries
     MISC_OVF_TOP = 8,          // overflow chunks taken
     MISC_BIG_COUNT = 9,        // rows routed to the wave-per-row kernel
+    MISC_MAX_BIG = 10,         // most entries in one of those rows
     MISC_SIZE = 16
 };
 
@@ -55,9 +56,18 @@ struct ListPool {
     int32_t* misc;
     int64_t n_nodes;           // home chunks
     int64_t cap_chunks;        // home + overflow capacity
-    int32_t C;                 // entries per chunk
+    int32_t C;                 // entries per chunk (<= ALLS_MAX_CHUNK)
     int32_t B;                 // values per entry
+    uint32_t cmagic;           // chunk_magic(C): s / C as a multiply-high
 };
+
+#define ALLS_MAX_CHUNK 16384
+
+// Multiplier for s / C without a divide: floor(s * m / 2^32) with
+// m = ceil(2^32 / C) equals floor(s / C) for s < 2^32 / C (>= 262144 slots
+// for C <= ALLS_MAX_CHUNK) and never falls below it for larger s, so a slot
+// past the 8-chunk directory is always seen as past it.
+inline uint32_t chunk_magic(int32_t C) { return 0xffffffffu / (uint32_t)C + 1u; }
 
 // Element -> CSR-slot scatter of a cached pattern.  map[e*npe2 + i*npe + j]
 // is the rank of column node conn[e][j] among the sorted unique column nodes
@@ -105,6 +115,8 @@ ALLS_DEV int64_t xcd_block(int64_t bid, int64_t nb) {
 }
 
 // Reserve n consecutive slots of node-row a; returns the first slot.
+ALLS_DEV int32_t chunk_of(const ListPool& p, int32_t s) { return (int32_t)__umulhi((uint32_t)s, p.cmagic); }
+
 ALLS_DEV int32_t list_reserve(const ListPool& p, int32_t a, int32_t n) {
     return add_agent(&p.hdr[(int64_t)a * ALLS_DIR], n);
 }
@@ -112,7 +124,7 @@ ALLS_DEV int32_t list_reserve(const ListPool& p, int32_t a, int32_t n) {
 // The reservation [pos, pos+n) owns every chunk k >= 1 whose first slot
 // k*C lies inside it: take that chunk from the pool and publish it.
 ALLS_DEV void list_own_chunks(const ListPool& p, int32_t a, int32_t pos, int32_t n) {
-    int32_t k = (pos + p.C - 1) / p.C;
+    int32_t k = chunk_of(p, pos + p.C - 1);
     if (k < 1) k = 1;
     for (; k * p.C < pos + n; ++k) {
         if (k >= ALLS_DIR) {
@@ -134,7 +146,7 @@ ALLS_DEV void list_own_chunks(const ListPool& p, int32_t a, int32_t pos, int32_t
 // Chunk index holding slot s of row a, or -1.  Waits (bounded) for the
 // owner of an overflow chunk to publish it.
 ALLS_DEV int64_t list_chunk(const ListPool& p, int32_t a, int32_t s) {
-    int32_t k = s / p.C;
+    int32_t k = chunk_of(p, s);
     if (k == 0) return a;
     if (k >= ALLS_DIR) return -1;
     const int32_t* d = &p.hdr[(int64_t)a * ALLS_DIR + k];
@@ -199,9 +211,12 @@ void launch_count(const ListPool& p, int small_group, int f32vals, int32_t* u, i
 void launch_scan_total(const int32_t* u, int64_t n, int64_t* tiles, int64_t ntiles, hipStream_t s);
 void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dpn, void* row_ptr,
                        int i64, hipStream_t s);
-void launch_write(const ListPool& p, int small_group, int f32vals, int dpn, const int32_t* big_list,
+// max_big: most entries in a row of the wave-per-row kernel (MISC_MAX_BIG).
+void launch_write(const ListPool& p, int small_group, int f32vals, int dpn, int32_t max_big, const int32_t* big_list,
                   const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s);
 int64_t scan_tile_count(int64_t n);
+// Most list entries one node row may hold (consolidation LDS limit).
+int32_t long_row_limit();
 
 }  // namespace alls
 #endif

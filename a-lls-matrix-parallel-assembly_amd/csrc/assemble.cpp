@@ -418,12 +418,14 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     const int64_t n_rows = n_nodes * dpn;
     int C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
     if (C < 4) C = 4;
+    if (C > ALLS_MAX_CHUNK) C = ALLS_MAX_CHUNK;
     const int G = small_group(mi.kind);
     const int64_t ntiles = scan_tile_count(n_nodes);
     int retries = 0;
     bool full_pool = false;
     ListPool lp{};
     int64_t total = 0;
+    int32_t max_big = 0;
     for (;;) {
         // Overflow chunks: the exact bound (entries / C) is rarely approached,
         // since most rows fit their home chunk; reserve a quarter of it plus a
@@ -446,6 +448,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
         lp.cap_chunks = cap_chunks;
         lp.C = C;
         lp.B = B;
+        lp.cmagic = chunk_magic(C);
 
         HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
         launch_fill_i32(lp.hdr, n_nodes * ALLS_DIR, 0, s);
@@ -482,13 +485,26 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
                 return ALLS_E_POOL_OVERFLOW;
             }
             if (need > 0 || f[MISC_ERR + ERR_ROW_TOO_BIG]) {
-                int nc = 2 * C;
-                while ((int64_t)nc * ALLS_DIR < need) nc *= 2;
-                C = nc;
+                int64_t nc = 2 * (int64_t)C;
+                while (nc * ALLS_DIR < need) nc *= 2;
+                if (nc > ALLS_MAX_CHUNK) {
+                    if (err)
+                        *err = "a node row needs " + std::to_string(need) + " list entries; at most " +
+                               std::to_string(long_row_limit()) + " are supported";
+                    return ALLS_E_UNSUPPORTED;
+                }
+                C = (int)nc;
             } else {
                 full_pool = true;
             }
             continue;
+        }
+        max_big = f[MISC_MAX_BIG];
+        if (max_big > long_row_limit()) {
+            if (err)
+                *err = "a node row has " + std::to_string(max_big) + " list entries; at most " +
+                       std::to_string(long_row_limit()) + " are supported";
+            return ALLS_E_UNSUPPORTED;
         }
         total = ws.host_misc[32];
         if (st) {
@@ -508,7 +524,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
     launch_scan_apply(ws.u.as<int32_t>(), n_nodes, ws.tiles.as<int64_t>(), dpn, out->row_ptr, i64, s);
     HIP_TRY(hipGetLastError(), "launch scan apply");
-    launch_write(lp, G, mi.f32 ? 1 : 0, dpn, ws.big.as<int32_t>(), out->row_ptr, i64, out->col_ind, out->val, s);
+    launch_write(lp, G, mi.f32 ? 1 : 0, dpn, max_big, ws.big.as<int32_t>(), out->row_ptr, i64, out->col_ind, out->val, s);
     HIP_TRY(hipGetLastError(), "launch write");
     HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
     out->pattern_id = 0;

@@ -11,10 +11,12 @@
 //           write col_ind / val of the dpn DOF rows of the node.
 // Rows of <= G entries use G lanes and a register bitonic sort through
 // cross-lane shuffles.  Longer rows (Hex20, or unstructured outliers) use a
-// wave per row: an LDS hash set for the count, a sort staged in LDS for the
-// write.  Sorting by (column, element id)
-// makes the summation order, and so the values, independent of the order in
-// which atomics appended the entries.
+// wave per row and LDS: a hash set of the columns for the count; for the
+// write, the same hash set, column ranks by counting, a bucket per column and
+// an element-id sort inside each bucket (a handful of entries), instead of a
+// full sort of the row.  Either way every column's contributions are summed
+// in ascending element id, so the values do not depend on the order in which
+// atomics appended the entries.
 #include "alls_internal.h"
 
 namespace {
@@ -30,11 +32,30 @@ __device__ __forceinline__ int64_t entry_index(const ListPool& p, int64_t a, con
     // Callers clamp s below ALLS_DIR * C.  A chunk that was never published
     // (only after a directory overflow, which the host then retries) maps to
     // the home chunk so that every access stays in bounds.
-    const int32_t k = s / p.C;
+    const int32_t k = chunk_of(p, s);
     const int32_t d = k == 0 ? 0 : h[k];
     const int64_t ch = d > 0 ? p.n_nodes + (int64_t)(d - 1) : a;
-    return ch * p.C + (s % p.C);
+    return ch * p.C + (s - k * p.C);
 }
+
+// entry_index with the row's directory already in registers (wave-uniform,
+// loaded once per row): no memory access per entry.
+struct RowDir {
+    int32_t d[ALLS_DIR];
+    __device__ __forceinline__ void load(const int32_t* h) {
+        const int4 x = reinterpret_cast<const int4*>(h)[0], y = reinterpret_cast<const int4*>(h)[1];
+        d[0] = x.x; d[1] = x.y; d[2] = x.z; d[3] = x.w;
+        d[4] = y.x; d[5] = y.y; d[6] = y.z; d[7] = y.w;
+    }
+    __device__ __forceinline__ int64_t entry(const ListPool& p, int64_t a, int32_t s) const {
+        const int32_t k = chunk_of(p, s);
+        int32_t dk = 0;
+#pragma unroll
+        for (int j = 1; j < ALLS_DIR; ++j) dk = k == j ? d[j] : dk;
+        const int64_t ch = dk > 0 ? p.n_nodes + (int64_t)(dk - 1) : a;
+        return ch * p.C + (s - k * p.C);
+    }
+};
 
 // ------------------------------------------------ register (small) rows ---
 template <int G, int B, typename VT, bool WRITE, typename RP>
@@ -131,7 +152,9 @@ __device__ __forceinline__ int32_t wave_excl_scan(int32_t x, int32_t* total) {
     return s - x;
 }
 
-// One 64-thread block (one wave) per row at a time; `cap` keys of LDS.
+// Rows beyond k_rows_merge's LDS (more than MERGE_CAP_MAX entries): one
+// 64-thread block (one wave) per row at a time, a bitonic sort of `cap` keys
+// staged in LDS.
 template <int B, typename VT, bool WRITE, typename RP>
 __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list, int32_t* u,
                                                  const RP* row_ptr, int32_t* col_ind, double* val,
@@ -213,6 +236,361 @@ __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list
     }
 }
 
+// Hash slot of column `col` in a table of 2^(32-shift) slots.
+__device__ __forceinline__ uint32_t col_hash(int32_t col, int shift) {
+    return ((uint32_t)col * 0x9E3779B1u) >> shift;
+}
+
+// Write pass of long rows without sorting them.  One 64-thread block (one
+// wave) per row at a time, `cap` (power of two, >= the longest row) sizes
+// the LDS arrays:
+//   1. load the row's keys, insert each column into a hash set (2*cap slots);
+//   2. number the occupied slots (unique index), which also gives u;
+//   3. count entries per unique column;
+//   4. rank of each column = number of smaller columns (counting over u);
+//   5. bucket offsets by rank (scan), entries dropped into their bucket;
+//   6. lane per column: insertion-sort its bucket by element id, sum the
+//      values in that order and write the dpn x dpn block; lanes take
+//      consecutive columns, so the stores are contiguous.
+template <int B, typename VT, typename RP>
+__global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* list, const RP* row_ptr,
+                                                   int32_t* col_ind, double* val, int cap) {
+    constexpr int D = B == 4 ? 2 : 3;
+    extern __shared__ int32_t smem_merge[];
+    const int ts = 2 * cap;
+    int32_t* table = smem_merge;       // [ts]   column, then unique index of the slot
+    int32_t* eeid = table + ts;        // [cap]  element id of entry i
+    int32_t* eu = eeid + cap;          // [cap]  table slot, then unique index of entry i
+    int32_t* ucol = eu + cap;          // [cap]  column of unique index q (padded to 4)
+    int32_t* ucnt = ucol + cap;        // [cap]  entries of unique index q
+    int32_t* urank = ucnt + cap;       // [cap]  rank of unique index q
+    int32_t* rcol = urank + cap;       // [cap]  column of rank r
+    int32_t* goff = rcol + cap;        // [cap+1] bucket start of rank r
+    int32_t* gfill = goff + cap + 1;   // [cap]  bucket fill of rank r
+    int32_t* memb = gfill + cap;       // [cap]  entries grouped by rank
+    const int lane = threadIdx.x;
+    const int shift = 32 - (31 - __clz(ts));
+    const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
+    for (int64_t row = xcd_block(blockIdx.x, gridDim.x); row < count; row += gridDim.x) {
+        const int64_t a = list ? (int64_t)list[row] : row;
+        const int32_t* h = p.hdr + a * ALLS_DIR;
+        const int32_t n = min(h[0], ALLS_DIR * p.C);
+        if (n > cap) {   // the host sizes cap from the count pass; never taken
+            if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], n);
+            continue;
+        }
+        for (int i = lane; i < ts; i += 64) table[i] = -1;
+        for (int i = lane; i < cap; i += 64) {
+            ucnt[i] = 0;
+            gfill[i] = 0;
+        }
+        __syncthreads();
+        // 1. keys -> hash set; eu[i] = slot of the entry's column.
+        for (int i = lane; i < n; i += 64) {
+            const int2 k = p.keys[entry_index(p, a, h, i)];
+            eeid[i] = k.y;
+            uint32_t slot = col_hash(k.x, shift);
+            for (int probe = 0; probe < ts; ++probe) {
+                const int32_t old = atomicCAS(&table[slot], -1, k.x);
+                if (old == -1 || old == k.x) break;
+                slot = (slot + 1) & (uint32_t)(ts - 1);
+            }
+            eu[i] = (int32_t)slot;
+        }
+        __syncthreads();
+        // 2. unique index = order of the occupied slot; the slot keeps it.
+        const int tseg = ts / 64;
+        int32_t occ = 0;
+        for (int j = 0; j < tseg; ++j) occ += table[lane * tseg + j] >= 0 ? 1 : 0;
+        int32_t u;
+        int32_t q = wave_excl_scan(occ, &u);
+        for (int j = 0; j < tseg; ++j) {
+            const int32_t c = table[lane * tseg + j];
+            if (c >= 0) {
+                ucol[q] = c;
+                table[lane * tseg + j] = q++;
+            }
+        }
+        const int u4 = (u + 3) & ~3;
+        if (lane < u4 - u) ucol[u + lane] = 0x7fffffff;
+        __syncthreads();
+        // 3. entries per unique column.
+        for (int i = lane; i < n; i += 64) {
+            const int32_t qi = table[eu[i]];
+            eu[i] = qi;
+            atomicAdd(&ucnt[qi], 1);
+        }
+        __syncthreads();
+        // 4. ranks.
+        for (int qi = lane; qi < u; qi += 64) {
+            const int32_t c = ucol[qi];
+            int32_t r = 0;
+            for (int k = 0; k < u4; k += 4) {
+                const int4 w = *reinterpret_cast<const int4*>(ucol + k);
+                r += (w.x < c) + (w.y < c) + (w.z < c) + (w.w < c);
+            }
+            urank[qi] = r;
+            rcol[r] = c;
+            goff[r] = ucnt[qi];
+        }
+        __syncthreads();
+        // 5. bucket offsets (exclusive scan over ranks), then the buckets.
+        {
+            const int rseg = (u + 63) / 64;
+            const int lo = min(lane * rseg, u), hi = min(lo + rseg, u);
+            int32_t sum = 0;
+            for (int r = lo; r < hi; ++r) sum += goff[r];
+            int32_t tot;
+            int32_t off = wave_excl_scan(sum, &tot);
+            for (int r = lo; r < hi; ++r) {
+                const int32_t c = goff[r];
+                goff[r] = off;
+                off += c;
+            }
+            if (lane == 0) goff[u] = n;
+        }
+        __syncthreads();
+        for (int i = lane; i < n; i += 64) {
+            const int32_t r = urank[eu[i]];
+            memb[goff[r] + atomicAdd(&gfill[r], 1)] = i;
+        }
+        __syncthreads();
+        // 6. one lane per column.
+        for (int r = lane; r < u; r += 64) {
+            const int32_t g0 = goff[r], g1 = goff[r + 1];
+            for (int m = g0 + 1; m < g1; ++m) {
+                const int32_t x = memb[m];
+                const int32_t ex = eeid[x];
+                int k = m - 1;
+                while (k >= g0 && eeid[memb[k]] > ex) {
+                    memb[k + 1] = memb[k];
+                    --k;
+                }
+                memb[k + 1] = x;
+            }
+            double v[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) v[b] = 0.0;
+            for (int m = g0; m < g1; ++m) {
+                const VT* src = static_cast<const VT*>(p.vals) + entry_index(p, a, h, memb[m]) * B;
+#pragma unroll
+                for (int b = 0; b < B; ++b) v[b] += (double)src[b];
+            }
+            const int32_t col = rcol[r];
+#pragma unroll
+            for (int dr = 0; dr < D; ++dr) {
+                const int64_t rs = (int64_t)row_ptr[a * D + dr] + (int64_t)r * D;
+#pragma unroll
+                for (int dc = 0; dc < D; ++dc) {
+                    col_ind[rs + dc] = col * D + dc;
+                    val[rs + dc] = v[dr * D + dc];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// k_rows_merge for rows of at most 64*EPL entries (EPL = entries per lane),
+// with the memory latency taken off the critical path: every lane issues
+// the loads of its entries' keys AND values at the top of the row, keeps
+// the values in registers while the LDS phases (hash set, ranks, buckets,
+// element-id sort) run, and then hands them over through LDS, one DOF row
+// (dpn values per entry) at a time, in bucket order.  One global-memory
+// round trip per row instead of one per bucket member.
+template <int B, typename VT, typename RP, int EPL>
+__global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t* list, const RP* row_ptr,
+                                                       int32_t* col_ind, double* val) {
+    constexpr int D = B == 4 ? 2 : 3;
+    constexpr int CAP = 64 * EPL;
+    constexpr int TS = 2 * CAP;
+    __shared__ __align__(16) int32_t sm[11 * CAP + 1];
+    // Phase arrays; the first 8*CAP ints are dead once the buckets are sorted
+    // and then hold the value stage (CAP * D doubles).
+    int32_t* table = sm;               // [TS]
+    int32_t* eeid = table + TS;        // [CAP]
+    int32_t* memb = eeid + CAP;        // [CAP]
+    int32_t* ucol = memb + CAP;        // [CAP]
+    int32_t* ucnt = ucol + CAP;        // [CAP]
+    int32_t* urank = ucnt + CAP;       // [CAP]
+    int32_t* gfill = urank + CAP;      // [CAP]
+    double* stage = reinterpret_cast<double*>(sm);
+    int32_t* eu = sm + 8 * CAP;        // [CAP]  slot, unique index, then bucket position of entry i
+    int32_t* goff = eu + CAP;          // [CAP+1]
+    int32_t* rcol = goff + CAP + 1;    // [CAP]
+    static_assert(CAP * D * 2 <= 8 * CAP, "value stage must fit the dead phase arrays");
+    const int lane = threadIdx.x;
+    constexpr int shift = 32 - (31 - __builtin_clz(TS));
+    const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
+    for (int64_t row = xcd_block(blockIdx.x, gridDim.x); row < count; row += gridDim.x) {
+        const int64_t a = list ? (int64_t)list[row] : row;
+        RowDir dir;
+        dir.load(p.hdr + a * ALLS_DIR);
+        const int32_t n = min(dir.d[0], ALLS_DIR * p.C);
+        if (n > CAP) {   // the host picks EPL from the count pass; never taken
+            if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], n);
+            continue;
+        }
+        int2 kk[EPL];
+        VT vv[EPL][B];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            kk[k] = make_int2(0, 0);
+#pragma unroll
+            for (int b = 0; b < B; ++b) vv[k][b] = VT(0);
+            if (i < n) {
+                const int64_t ei = dir.entry(p, a, i);
+                kk[k] = p.keys[ei];
+                const VT* src = static_cast<const VT*>(p.vals) + ei * B;
+#pragma unroll
+                for (int b = 0; b < B; ++b) vv[k][b] = src[b];
+            }
+        }
+        for (int i = lane; i < TS; i += 64) table[i] = -1;
+        for (int i = lane; i < CAP; i += 64) {
+            ucnt[i] = 0;
+            gfill[i] = 0;
+        }
+        __syncthreads();
+        // Hash set of the columns; eu[i] = slot of the entry's column.
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            if (i < n) {
+                eeid[i] = kk[k].y;
+                uint32_t slot = col_hash(kk[k].x, shift);
+                for (int probe = 0; probe < TS; ++probe) {
+                    const int32_t old = atomicCAS(&table[slot], -1, kk[k].x);
+                    if (old == -1 || old == kk[k].x) break;
+                    slot = (slot + 1) & (uint32_t)(TS - 1);
+                }
+                eu[i] = (int32_t)slot;
+            }
+        }
+        __syncthreads();
+        // Unique index = order of the occupied slot.
+        constexpr int TSEG = TS / 64;
+        int32_t occ = 0;
+#pragma unroll
+        for (int j = 0; j < TSEG; ++j) occ += table[lane * TSEG + j] >= 0 ? 1 : 0;
+        int32_t u;
+        int32_t q = wave_excl_scan(occ, &u);
+#pragma unroll
+        for (int j = 0; j < TSEG; ++j) {
+            const int32_t c = table[lane * TSEG + j];
+            if (c >= 0) {
+                ucol[q] = c;
+                table[lane * TSEG + j] = q++;
+            }
+        }
+        const int u4 = (u + 3) & ~3;
+        if (lane < u4 - u) ucol[u + lane] = 0x7fffffff;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            if (i < n) {
+                const int32_t qi = table[eu[i]];
+                eu[i] = qi;
+                atomicAdd(&ucnt[qi], 1);
+            }
+        }
+        __syncthreads();
+        // Ranks by counting smaller columns.
+        for (int qi = lane; qi < u; qi += 64) {
+            const int32_t c = ucol[qi];
+            int32_t r = 0;
+            for (int k = 0; k < u4; k += 4) {
+                const int4 w = *reinterpret_cast<const int4*>(ucol + k);
+                r += (w.x < c) + (w.y < c) + (w.z < c) + (w.w < c);
+            }
+            urank[qi] = r;
+            rcol[r] = c;
+            goff[r] = ucnt[qi];
+        }
+        __syncthreads();
+        {
+            const int rseg = (u + 63) / 64;
+            const int lo = min(lane * rseg, u), hi = min(lo + rseg, u);
+            int32_t sum = 0;
+            for (int r = lo; r < hi; ++r) sum += goff[r];
+            int32_t tot;
+            int32_t off = wave_excl_scan(sum, &tot);
+            for (int r = lo; r < hi; ++r) {
+                const int32_t c = goff[r];
+                goff[r] = off;
+                off += c;
+            }
+            if (lane == 0) goff[u] = n;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            if (i < n) {
+                const int32_t r = urank[eu[i]];
+                memb[goff[r] + atomicAdd(&gfill[r], 1)] = i;
+            }
+        }
+        __syncthreads();
+        // Element-id order inside each bucket; eu[i] = position of entry i.
+        for (int r = lane; r < u; r += 64) {
+            const int32_t g0 = goff[r], g1 = goff[r + 1];
+            for (int m = g0 + 1; m < g1; ++m) {
+                const int32_t x = memb[m];
+                const int32_t ex = eeid[x];
+                int k = m - 1;
+                while (k >= g0 && eeid[memb[k]] > ex) {
+                    memb[k + 1] = memb[k];
+                    --k;
+                }
+                memb[k + 1] = x;
+            }
+            for (int m = g0; m < g1; ++m) eu[memb[m]] = m;
+        }
+        __syncthreads();
+        // One DOF row at a time: stage the entries' dpn values at their
+        // bucket positions, then a lane per column sums its bucket in order.
+#pragma unroll
+        for (int dr = 0; dr < D; ++dr) {
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) {
+                const int i = lane + 64 * k;
+                if (i < n) {
+                    const int32_t pos = eu[i];
+#pragma unroll
+                    for (int dc = 0; dc < D; ++dc) stage[pos * D + dc] = (double)vv[k][dr * D + dc];
+                }
+            }
+            __syncthreads();
+            const int64_t rbase = (int64_t)row_ptr[a * D + dr];
+            for (int r = lane; r < u; r += 64) {
+                double acc[D];
+#pragma unroll
+                for (int dc = 0; dc < D; ++dc) acc[dc] = 0.0;
+                for (int m = goff[r]; m < goff[r + 1]; ++m) {
+#pragma unroll
+                    for (int dc = 0; dc < D; ++dc) acc[dc] += stage[m * D + dc];
+                }
+                const int64_t rs = rbase + (int64_t)r * D;
+                const int32_t col = rcol[r];
+#pragma unroll
+                for (int dc = 0; dc < D; ++dc) {
+                    col_ind[rs + dc] = col * D + dc;
+                    val[rs + dc] = acc[dc];
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// LDS bytes of k_rows_merge for `cap`.
+constexpr size_t merge_lds(int cap) { return (size_t)(2 * cap + 9 * cap + 1) * sizeof(int32_t); }
+constexpr int MERGE_CAP_MAX = 1024;   // 45 KB of LDS
+constexpr int LONG_CAP_MAX = 4096;    // k_rows_big: 40 KB of LDS
+
 // Unique-column count of long rows with an LDS hash set (one 64-thread block
 // per row at a time, `ts` table slots, ts >= 2 * ALLS_DIR * C).  Counting
 // needs no order, so no sort: each entry inserts its column with one LDS
@@ -222,16 +600,19 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
     const int lane = threadIdx.x;
     const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
     const int shift = 32 - (31 - __clz(ts));
+    int32_t longest = 0;   // this block's longest row; one atomic per block
     for (int64_t r = blockIdx.x; r < count; r += gridDim.x) {
         const int64_t a = list ? (int64_t)list[r] : r;
-        const int32_t* h = p.hdr + a * ALLS_DIR;
-        const int32_t n = min(h[0], ALLS_DIR * p.C);
+        RowDir dir;
+        dir.load(p.hdr + a * ALLS_DIR);
+        const int32_t n = min(dir.d[0], ALLS_DIR * p.C);
+        longest = max(longest, n);
         for (int i = lane; i < ts; i += 64) table[i] = -1;
         __syncthreads();
         int32_t mine = 0;
         for (int i = lane; i < n; i += 64) {
-            const int32_t col = p.keys[entry_index(p, a, h, i)].x;
-            uint32_t slot = ((uint32_t)col * 0x9E3779B1u) >> shift;
+            const int32_t col = p.keys[dir.entry(p, a, i)].x;
+            uint32_t slot = col_hash(col, shift);
             for (int probe = 0; probe < ts; ++probe) {
                 const int32_t old = atomicCAS(&table[slot], -1, col);
                 if (old == -1) {
@@ -247,6 +628,8 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
         if (lane == 0) u[a] = total;
         __syncthreads();
     }
+    // Reading first keeps the shared counter from serialising every block.
+    if (lane == 0 && longest > ld_agent(&p.misc[MISC_MAX_BIG])) atomicMax(&p.misc[MISC_MAX_BIG], longest);
 }
 
 // ------------------------------------------------------------- scan ---
@@ -327,7 +710,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int32_t* u, i
 }
 
 template <int B, typename VT, bool WRITE, typename RP>
-void rows_dispatch(const ListPool& p, int G, const int32_t* big_list, int32_t* u, int32_t* bl,
+void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big_list, int32_t* u, int32_t* bl,
                    const RP* rp, int32_t* ci, double* v, int dpn, hipStream_t s) {
     if (G > 0) {
         const int64_t threads = p.n_nodes * G;
@@ -340,17 +723,40 @@ void rows_dispatch(const ListPool& p, int G, const int32_t* big_list, int32_t* u
             hipLaunchKernelGGL((k_rows_small<64, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn);
     }
     // Long rows: over the big-row list (G > 0) or over every row (G == 0).
-    int cap = 1;
-    while (cap < ALLS_DIR * p.C) cap <<= 1;
     int64_t grid = G > 0 ? 2048 : p.n_nodes;
     if (grid > 262144) grid = 262144;
     if (grid < 1) grid = 1;
     if (!WRITE) {
+        int cap = 1;
+        while (cap < ALLS_DIR * p.C) cap <<= 1;
         const int ts = 2 * cap;
         hipLaunchKernelGGL(k_rows_hash_count, dim3((unsigned)grid), dim3(64), (size_t)ts * sizeof(int32_t), s, p,
                            G > 0 ? big_list : nullptr, u, ts);
         return;
     }
+    if (max_big <= 0) return;   // no long rows
+    int cap = 64;
+    while (cap < max_big) cap <<= 1;
+    const int32_t* bl_or_all = G > 0 ? big_list : nullptr;
+    if (cap <= 256) {
+        if (cap == 64)
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+                               rp, ci, v);
+        else if (cap == 128)
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 2>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+                               rp, ci, v);
+        else
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+                               rp, ci, v);
+        return;
+    }
+    if (cap <= MERGE_CAP_MAX) {
+        hipLaunchKernelGGL((k_rows_merge<B, VT, RP>), dim3((unsigned)grid), dim3(64), merge_lds(cap), s, p,
+                           bl_or_all, rp, ci, v, cap);
+        return;
+    }
+    // Rows of more than MERGE_CAP_MAX entries (a node shared by very many
+    // elements): sort staged in LDS, up to LONG_CAP_MAX entries.
     const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(uint16_t));
     hipLaunchKernelGGL((k_rows_big<B, VT, WRITE, RP>), dim3((unsigned)grid), dim3(64), lds, s, p,
                        G > 0 ? big_list : nullptr, u, rp, ci, v, dpn, cap);
@@ -360,15 +766,17 @@ void rows_dispatch(const ListPool& p, int G, const int32_t* big_list, int32_t* u
 
 namespace alls {
 
+int32_t long_row_limit() { return LONG_CAP_MAX; }
+
 int64_t scan_tile_count(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 void launch_count(const ListPool& p, int G, int f32vals, int32_t* u, int32_t* big_list, hipStream_t s) {
     if (p.B == 4)
-        rows_dispatch<4, double, false, int32_t>(p, G, big_list, u, big_list, nullptr, nullptr, nullptr, 2, s);
+        rows_dispatch<4, double, false, int32_t>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 2, s);
     else if (f32vals)
-        rows_dispatch<9, float, false, int32_t>(p, G, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
+        rows_dispatch<9, float, false, int32_t>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
     else
-        rows_dispatch<9, double, false, int32_t>(p, G, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
+        rows_dispatch<9, double, false, int32_t>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
 }
 
 void launch_scan_total(const int32_t* u, int64_t n, int64_t* tiles, int64_t ntiles, hipStream_t s) {
@@ -393,19 +801,20 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
                            static_cast<int32_t*>(row_ptr));
 }
 
-void launch_write(const ListPool& p, int G, int f32vals, int dpn, const int32_t* big_list, const void* row_ptr,
-                  int i64, int32_t* col_ind, double* val, hipStream_t s) {
+void launch_write(const ListPool& p, int G, int f32vals, int dpn, int32_t max_big, const int32_t* big_list,
+                  const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s) {
     (void)dpn;
+    const int32_t mb = max_big;
     if (i64) {
         const int64_t* rp = static_cast<const int64_t*>(row_ptr);
-        if (p.B == 4) rows_dispatch<4, double, true, int64_t>(p, G, big_list, nullptr, nullptr, rp, col_ind, val, 2, s);
-        else if (f32vals) rows_dispatch<9, float, true, int64_t>(p, G, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
-        else rows_dispatch<9, double, true, int64_t>(p, G, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
+        if (p.B == 4) rows_dispatch<4, double, true, int64_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 2, s);
+        else if (f32vals) rows_dispatch<9, float, true, int64_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
+        else rows_dispatch<9, double, true, int64_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
     } else {
         const int32_t* rp = static_cast<const int32_t*>(row_ptr);
-        if (p.B == 4) rows_dispatch<4, double, true, int32_t>(p, G, big_list, nullptr, nullptr, rp, col_ind, val, 2, s);
-        else if (f32vals) rows_dispatch<9, float, true, int32_t>(p, G, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
-        else rows_dispatch<9, double, true, int32_t>(p, G, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
+        if (p.B == 4) rows_dispatch<4, double, true, int32_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 2, s);
+        else if (f32vals) rows_dispatch<9, float, true, int32_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
+        else rows_dispatch<9, double, true, int32_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
     }
 }
 

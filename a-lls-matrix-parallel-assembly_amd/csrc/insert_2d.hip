@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
             if (ch < 0) continue;
             double k[4];
             el.block(i, j, k);
-            store_entry(p, ch * p.C + (s % p.C), el.nd[j], (int32_t)e, k);
+            store_entry(p, ch * p.C + (s - chunk_of(p, s) * p.C), el.nd[j], (int32_t)e, k);
         }
     }
 }

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(64) void k_hex20(DevMesh m, ListPool p, Scatter sc,
             const int32_t s = posv[I] + Jn;
             const int64_t ch = list_chunk(p, nodes[I], s);
             if (ch < 0) continue;
-            const int64_t id = ch * p.C + (s % p.C);
+            const int64_t id = ch * p.C + (s - chunk_of(p, s) * p.C);
             T* dst = static_cast<T*>(p.vals) + id * 9;
 #pragma unroll
             for (int r = 0; r < 3; ++r)

@@ -270,7 +270,7 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
     if (hipSetDevice(cfg->device) != hipSuccess) return ALLS_E_HIP;
     auto* s = new alls_server();
     s->cfg = *cfg;
-    const int n = std::min(std::max(cfg->n_streams, 1), 16);
+    const int n = cfg->n_streams <= 0 ? 2 : std::min(cfg->n_streams, 16);
     s->cfg.n_streams = n;
     if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 1 << 20;   // bounded by the budget
     size_t free_b = 0, total_b = 0;

@@ -222,12 +222,12 @@ def served_mix(n_requests, costs):
     t_gen = time.perf_counter() - t0
     busy_s = serve_mix.offered_busy_s(reqs, costs)
     rate = 0.75 * n_requests / busy_s
-    with alls.Server(n_streams=4) as srv:
+    with alls.Server(n_streams=2) as srv:
         stats, makespan, errors = serve_mix.run(srv, reqs, rate, progress=log)
     res = serve_mix.summarize(reqs, stats, makespan, errors)
     res.update({"arrival_rate_per_s": round(rate, 1), "unique_meshes": len(meshes),
                 "offered_busy_s": round(busy_s, 2), "cost_model_ms_per_element": costs,
-                "generation_s": round(t_gen, 1), "streams": 4,
+                "generation_s": round(t_gen, 1), "streams": 2,
                 "workload": "Poisson arrivals; type uniform over {Q4, T3, Hex20}; size log-uniform 1e3-1e6 "
                             "elements; 60% reuse a seen mesh with a new lognormal(0,0.5) E (cached pattern)"})
     return res

@@ -79,7 +79,7 @@ typedef struct {
 
 typedef struct {
     int32_t precision;        /* alls_precision (Hex20 only) */
-    int32_t chunk_slots;      /* entries per list-pool chunk; 0 => tuned default */
+    int32_t chunk_slots;      /* entries per list-pool chunk; 0 => tuned default; at most 16384 */
     int32_t block_threads;    /* threads per block of the insert kernel; 0 => tuned default */
     int32_t row_ptr_i64;      /* 1 => always int64 row_ptr; otherwise int64 only when nnz >= 2^31 */
     int32_t timing;           /* 1 => time each stage with hipEvents (alls_last_stats) */
@@ -200,7 +200,8 @@ alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshe
  * patterns when mesh_key repeats. */
 typedef struct {
     int32_t device;
-    int32_t n_streams;        /* concurrent in-flight requests (<= 4 hardware queues by default) */
+    int32_t n_streams;        /* concurrent in-flight requests; 0 => 2 (more streams share the
+                                 device's 4 hardware queues and lose to allocation stalls) */
     int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM */
     int32_t pattern_cache_entries;
     int32_t reserved;

@@ -94,17 +94,31 @@ def test_small_chunks_force_overflow_retry(ctx):
     compare_csr(ctx.assemble(m2, chunk_slots=4).download(), oracle.assemble(m2))
 
 
-def test_big_row_fallback_for_unstructured_valence(ctx):
-    # A fan of 40 triangles around one node: that row gets 120 entries, more
-    # than the 32-lane register path holds, so it goes to the LDS kernel.
-    k = 40
+def fan_mesh(k):
+    """k triangles around node 0: that row gets 3k list entries."""
     ang = np.linspace(0, 2 * np.pi, k, endpoint=False)
     coords = np.concatenate([[0.0, 0.0], np.column_stack([np.cos(ang), np.sin(ang)]).reshape(-1)])
     conn = np.array([[0, 1 + i, 1 + (i + 1) % k] for i in range(k)], np.int32)
-    m = alls.MeshArrays(coords, conn, 2, 3, uniform_type=alls.T3)
+    return alls.MeshArrays(coords, conn, 2, 3, uniform_type=alls.T3)
+
+
+@pytest.mark.parametrize("k", [40, 300, 700])
+def test_big_row_fallback_for_unstructured_valence(ctx, k):
+    # 120 entries: more than the 32-lane register path holds, so the row goes
+    # to the LDS hash/bucket kernel; 900 entries: same kernel at its largest
+    # LDS size; 2100 entries: beyond it, the LDS sort kernel.
+    m = fan_mesh(k)
     out = ctx.assemble(m)
     assert ctx.stats()["big_rows"] >= 1
     compare_csr(out.download(), oracle.assemble(m))
+
+
+def test_row_beyond_consolidation_limit_is_refused(ctx):
+    # 1400 triangles on one node: 4200 entries in one node row, more than the
+    # 4096 the consolidation holds in LDS -> a clear error before the write pass.
+    with pytest.raises(alls.AllsError) as ei:
+        ctx.assemble(fan_mesh(1400))
+    assert ei.value.status == alls.E_UNSUPPORTED
 
 
 def test_empty_rows_and_orphan_nodes(ctx):
