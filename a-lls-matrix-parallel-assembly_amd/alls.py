@@ -432,6 +432,11 @@ class _Hip:
             if r != 0:
                 raise RuntimeError(f"hipMemcpy failed: {r}")
 
+    def dtod(self, dst, src, n):
+        r = self.L.hipMemcpy(dst, src, int(n), 3)   # hipMemcpyDeviceToDevice
+        if r != 0:
+            raise RuntimeError(f"hipMemcpy failed: {r}")
+
     def free(self, p):
         if p:
             self.L.hipFree(p)

@@ -97,6 +97,24 @@ def roofline(mesh, st, entry_bytes):
     }, kb, times
 
 
+# Kernel behind each pipeline stage of the Q4 workload (rocprofv3 names).
+STAGE_KERNEL = {"insert": "k_insert_2d<0>", "count": "k_rows_small<16, 4, double, false, int>",
+                "write": "k_rows_small<16, 4, double, true, int>"}
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (same workload), already scaled by the
+    copy-kernel calibration; None when the summary is absent."""
+    path = os.path.join(REPO, "profiles", "alls_bench_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t["kernels"][STAGE_KERNEL[stage]]["traffic_bytes"], t["source"]
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
 def cpu_baseline(n):
     """Oracle (serial restatement of Acc:329-380 + sorted pattern) on one core."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -244,6 +262,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="end with one device-to-device copy of BYTES (a copy kernel of known size, the "
+                         "scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built.",
@@ -260,6 +281,7 @@ def main():
     ms = dt / a.steps * 1e3
     value = mesh.n_elems * a.steps / dt
     dom, roof, kb, times = roofline(mesh, st, 8 + 4 * 8)
+    roof["traffic"], traffic_src = measured_traffic(dom)
     pb = path_bytes(mesh, st["nnz"])
     log(f"{value:.3e} elements/s, {ms:.3f} ms/step, stages {times}")
     out.free()
@@ -273,6 +295,7 @@ def main():
                    "n_elems": mesh.n_elems, "n_dofs": mesh.n_dofs, "nnz": st["nnz"]},
         "roofline": dict(roof, kernel=dom,
                          algorithmic_bytes_per_launch=kb[dom],
+                         traffic_source=traffic_src,
                          note="algorithmic bytes: mesh reads + list entries (insert); keys+values read and "
                               "CSR written (write)"),
         "stages_ms": {k: round(v, 4) for k, v in times.items()},
@@ -307,6 +330,14 @@ def main():
             extra["served_mix"] = {"error": str(e)}
     if extra:
         res["extra"] = extra
+    if a.calibrate_copy > 0:
+        h = alls.hip()
+        src, dst = h.malloc(a.calibrate_copy), h.malloc(a.calibrate_copy)
+        h.dtod(dst, src, a.calibrate_copy)
+        h.sync()
+        h.free(src)
+        h.free(dst)
+        res["calibration_copy_bytes"] = a.calibrate_copy
     print(json.dumps(res), flush=True)
 
 
