@@ -33,7 +33,8 @@ class Mesh(C.Structure):
 class Opts(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("chunk_slots", C.c_int32), ("block_threads", C.c_int32),
-        ("row_ptr_i64", C.c_int32), ("timing", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("row_ptr_i64", C.c_int32), ("timing", C.c_int32), ("grid_per_cu", C.c_int32),
+        ("autotune", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -111,7 +112,8 @@ SIGNATURES = {
     "alls_coloring_download": (C.c_int, [vp, vp, vp]),
     "alls_coloring_free": (None, [vp, vp]),
     "alls_assemble_colored": (C.c_int, [vp, vp, vp, vp, C.c_int32, P(Csr)]),
-    "alls_assemble_batch": (C.c_int, [vp, C.c_int32, P(Mesh), P(Opts), P(Csr)]),
+    "alls_assemble_batch": (C.c_int, [vp, C.c_int32, P(Mesh), P(Opts), P(Csr), P(C.c_int64), P(C.c_int64)]),
+    "alls_autotune": (C.c_int, [vp, P(Mesh), P(Opts), P(Opts)]),
     "alls_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
     "alls_server_submit": (C.c_int, [vp, P(Mesh), C.c_uint64, P(Opts), P(C.c_int64)]),
     "alls_server_wait": (C.c_int, [vp, C.c_int64, P(Csr), P(RequestStats)]),
@@ -266,15 +268,54 @@ class Context:
         return s.as_dict()
 
     def assemble(self, mesh, precision=PREC_F64, chunk_slots=0, block_threads=0, row_ptr_i64=0,
-                 timing=1, out=None, c_mesh=None):
+                 timing=1, out=None, c_mesh=None, grid_per_cu=0, autotune=0):
         """Plain A-LLS assembly; returns a DeviceCsr (reused if `out` is given)."""
         o = Opts()
         o.precision, o.chunk_slots, o.block_threads = precision, chunk_slots, block_threads
         o.row_ptr_i64, o.timing = row_ptr_i64, timing
+        o.grid_per_cu, o.autotune = grid_per_cu, autotune
         out = out or DeviceCsr(self)
         m = c_mesh if c_mesh is not None else mesh.c_mesh()
         check(lib().alls_assemble(self.handle, C.byref(m), C.byref(o), C.byref(out.csr)), self)
         return out
+
+
+    def assemble_batch(self, meshes, precision=PREC_F64, chunk_slots=0, block_threads=0, grid_per_cu=0,
+                       autotune=0, out=None, c_meshes=None):
+        """alls_assemble_batch: one pipeline over all meshes (same dim).  Returns
+        (DeviceCsr of the block-diagonal result, row_off, nnz_off), offsets as
+        numpy int64 arrays of len(meshes)+1."""
+        n = len(meshes) if c_meshes is None else len(c_meshes)
+        arr = (Mesh * n)(*[(c_meshes[k] if c_meshes is not None else meshes[k].c_mesh()) for k in range(n)])
+        o = Opts()
+        o.precision, o.chunk_slots, o.block_threads = precision, chunk_slots, block_threads
+        o.grid_per_cu, o.autotune, o.timing = grid_per_cu, autotune, 1
+        out = out or DeviceCsr(self)
+        row_off = np.zeros(n + 1, np.int64)
+        nnz_off = np.zeros(n + 1, np.int64)
+        check(lib().alls_assemble_batch(self.handle, n, arr, C.byref(o), C.byref(out.csr),
+                                        row_off.ctypes.data_as(P(C.c_int64)), nnz_off.ctypes.data_as(P(C.c_int64))),
+              self)
+        return out, row_off, nnz_off
+
+    def autotune(self, mesh, precision=PREC_F64, c_mesh=None):
+        """alls_autotune: the fastest launch setting for this mesh's kind and size class."""
+        base, best = Opts(), Opts()
+        base.precision, base.timing = precision, 1
+        m = c_mesh if c_mesh is not None else mesh.c_mesh()
+        check(lib().alls_autotune(self.handle, C.byref(m), C.byref(base), C.byref(best)), self)
+        return {"block_threads": best.block_threads, "chunk_slots": best.chunk_slots,
+                "grid_per_cu": best.grid_per_cu}
+
+
+def split_batch(csr, row_off, nnz_off):
+    """Per-mesh CSR triples (local row/column ids) from a downloaded batch result."""
+    rp, ci, v = csr
+    res = []
+    for k in range(len(row_off) - 1):
+        r0, r1, z0, z1 = row_off[k], row_off[k + 1], nnz_off[k], nnz_off[k + 1]
+        res.append((rp[r0:r1 + 1] - z0, ci[z0:z1] - r0, v[z0:z1]))
+    return res
 
 
 class Pattern:

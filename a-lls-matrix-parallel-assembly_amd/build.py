@@ -17,7 +17,8 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "liballs.so")
 BUILD = os.path.join(HERE, "_build")
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", os.path.join(ROCM, "bin", "hipcc"))
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
@@ -39,7 +40,7 @@ def compile_one(src, verbose):
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip"] + HIP_FLAGS + COMMON + ["-c", path, "-o", obj]
     else:
-        cmd = [HIPCC] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
+        cmd = [HIPCC, "-x", "c++", "-I", os.path.join(ROCM, "include")] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-c", path, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -174,8 +174,10 @@ namespace alls {
 
 // Kernel launchers (defined in the .hip files).
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
-void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, hipStream_t s);
-void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, hipStream_t s);
+// max_blocks > 0 caps the grid (persistent, grid-stride); 0 => one thread
+// (2D) or one block (Hex20) per element.
+void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, int64_t max_blocks, hipStream_t s);
+void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, int64_t max_blocks, hipStream_t s);
 // Pattern cache: build the scatter map; value-only and coloured scatter.
 void launch_build_map(const DevMesh& m, int kind, const void* row_ptr, int i64, const int32_t* col_ind,
                       int32_t* map, int32_t* misc, hipStream_t s);
@@ -204,6 +206,23 @@ void launch_conflicts(const DevMesh& m, const int64_t* off, const int32_t* list,
 void launch_color_count(const int32_t* col, int64_t n, int32_t* cnt, hipStream_t s);
 void launch_color_bucket(const int32_t* col, int64_t n, const int64_t* coff, int32_t* cursor, int32_t* elist,
                          hipStream_t s);
+// One source mesh of a batch (device arrays) and its place in the packed mesh.
+struct PackMesh {
+    const double* coords;
+    const int32_t* conn;
+    const uint8_t* types;      // or null: every element is `type`
+    const double* E_elem;      // or null: every element has E
+    double E;
+    int64_t n_nodes, n_elems, node_off, elem_off;
+    int32_t npe;               // conn stride of the source
+    int32_t type;
+};
+// Packs nm device meshes into coords/conn (and types, E when non-null).
+void launch_pack(const PackMesh* pm, int nm, int dim, int npe, int64_t n_nodes, int64_t n_elems, double* coords,
+                 int32_t* conn, uint8_t* types, double* E, hipStream_t s);
+// out[i] = row_ptr[idx[i]] as int64.
+void launch_gather_rp(const void* row_ptr, int i64, const int64_t* idx, int n, int64_t* out, hipStream_t s);
+
 // Consolidation.  small_group = lanes per row of the register kernel (0 =>
 // every row goes to the wave-per-row LDS kernel).
 void launch_count(const ListPool& p, int small_group, int f32vals, int32_t* u, int32_t* big_list,

@@ -161,6 +161,27 @@ hipError_t Workspace::init() {
     return hipSuccess;
 }
 
+hipError_t Workspace::staging(size_t total, char** host) {
+    *host = nullptr;
+    // The previous upload from this buffer has completed: every pipeline
+    // synchronises its stream before returning.
+    if (total > pinned_bytes) {
+        if (pinned) (void)hipHostFree(pinned);
+        pinned = nullptr;
+        pinned_bytes = 0;
+        const size_t nb = total + total / 4;
+        hipError_t e = hipHostMalloc(&pinned, nb, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            pinned = nullptr;
+            (void)hipGetLastError();
+            return e;
+        }
+        pinned_bytes = nb;
+    }
+    *host = static_cast<char*>(pinned);
+    return hipSuccess;
+}
+
 hipError_t Workspace::upload(void* const* dst, const void* const* src, const size_t* bytes, int n, hipStream_t s) {
     size_t total = 0;
     for (int i = 0; i < n; ++i) total += (bytes[i] + 255) & ~size_t(255);
@@ -194,16 +215,23 @@ hipError_t Workspace::upload(void* const* dst, const void* const* src, const siz
 
 size_t Workspace::device_bytes() const {
     size_t b = 0;
-    for (const DevBuf* x : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b += x->bytes;
+    for (const DevBuf* x : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc})
+        b += x->bytes;
     return b;
 }
 
 void Workspace::release_device() {
-    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b->release();
+    g_front.reset();
+    g_back.reset();
+    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc})
+        b->release();
 }
 
 void Workspace::destroy() {
-    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E}) b->release();
+    g_front.reset();
+    g_back.reset();
+    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc})
+        b->release();
     if (host_misc) (void)hipHostFree(host_misc);
     host_misc = nullptr;
     if (pinned) (void)hipHostFree(pinned);
@@ -408,141 +436,295 @@ static float ev_ms(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
-                     hipStream_t s, alls_stats* st, std::string* err) {
-    HIP_TRY(ws.init(), "workspace init");
-    const int dpn = mi.dpn;
-    const int B = dpn * dpn;
+// Grid cap of the insert kernel: opts.grid_per_cu blocks per CU (persistent,
+// grid-stride), or 0 for one thread (2D) / one block (Hex20) per element.
+static int64_t insert_max_blocks(int grid_per_cu) {
+    if (grid_per_cu <= 0) return 0;
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cus = n;
+    }
+    return (int64_t)grid_per_cu * cus;
+}
+
+// State carried across the pipeline's one host synchronisation.
+struct Pipeline {
+    int C = 0, G = 0;
+    int64_t ntiles = 0, cap_chunks = 0;
+    bool full_pool = false;
+    int retries = 0;
+    ListPool lp{};
+    int64_t total = 0;     // unique (row node, column node) pairs
+    int32_t max_big = 0;   // longest row of the wave-per-row kernels
+};
+
+// List pool for pl.C.  The exact bound of overflow chunks (entries / C) is
+// rarely approached, since most rows fit their home chunk: reserve a quarter
+// of it plus a chunk per 4 rows, and the full bound after a pool overflow.
+static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, Pipeline& pl,
+                               std::string* err) {
+    const int B = mi.dpn * mi.dpn;
     const size_t vsz = mi.f32 ? sizeof(float) : sizeof(double);
     const int64_t n_nodes = dm.n_nodes;
-    const int64_t n_rows = n_nodes * dpn;
-    int C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
-    if (C < 4) C = 4;
-    if (C > ALLS_MAX_CHUNK) C = ALLS_MAX_CHUNK;
-    const int G = small_group(mi.kind);
-    const int64_t ntiles = scan_tile_count(n_nodes);
-    int retries = 0;
-    bool full_pool = false;
-    ListPool lp{};
-    int64_t total = 0;
-    int32_t max_big = 0;
-    for (;;) {
-        // Overflow chunks: the exact bound (entries / C) is rarely approached,
-        // since most rows fit their home chunk; reserve a quarter of it plus a
-        // chunk per 4 rows and retry with the full bound if the pool runs out.
-        const int64_t bound = (mi.entries_bound + C - 1) / C + 1;
-        const int64_t ovf = full_pool ? bound : std::min(bound, bound / 4 + n_nodes / 4 + 1024);
-        const int64_t cap_chunks = n_nodes + ovf;
-        HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
-        HIP_TRY(ws.keys.ensure((size_t)cap_chunks * C * sizeof(int2)), "alloc list keys");
-        HIP_TRY(ws.vals.ensure((size_t)cap_chunks * C * B * vsz), "alloc list values");
-        HIP_TRY(ws.misc.ensure(MISC_SIZE * sizeof(int32_t)), "alloc misc");
-        HIP_TRY(ws.u.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc row counts");
-        HIP_TRY(ws.big.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc big-row list");
-        HIP_TRY(ws.tiles.ensure((size_t)(ntiles + 1) * sizeof(int64_t)), "alloc scan tiles");
-        lp.hdr = ws.hdr.as<int32_t>();
-        lp.keys = ws.keys.as<int2>();
-        lp.vals = ws.vals.p;
-        lp.misc = ws.misc.as<int32_t>();
-        lp.n_nodes = n_nodes;
-        lp.cap_chunks = cap_chunks;
-        lp.C = C;
-        lp.B = B;
-        lp.cmagic = chunk_magic(C);
+    const int64_t bound = (mi.entries_bound + pl.C - 1) / pl.C + 1;
+    const int64_t ovf = pl.full_pool ? bound : std::min(bound, bound / 4 + n_nodes / 4 + 1024);
+    pl.cap_chunks = n_nodes + ovf;
+    HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
+    HIP_TRY(ws.keys.ensure((size_t)pl.cap_chunks * pl.C * sizeof(int2)), "alloc list keys");
+    HIP_TRY(ws.vals.ensure((size_t)pl.cap_chunks * pl.C * B * vsz), "alloc list values");
+    HIP_TRY(ws.misc.ensure(MISC_SIZE * sizeof(int32_t)), "alloc misc");
+    HIP_TRY(ws.u.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc row counts");
+    HIP_TRY(ws.big.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc big-row list");
+    HIP_TRY(ws.tiles.ensure((size_t)(pl.ntiles + 1) * sizeof(int64_t)), "alloc scan tiles");
+    pl.lp.hdr = ws.hdr.as<int32_t>();
+    pl.lp.keys = ws.keys.as<int2>();
+    pl.lp.vals = ws.vals.p;
+    pl.lp.misc = ws.misc.as<int32_t>();
+    pl.lp.n_nodes = n_nodes;
+    pl.lp.cap_chunks = pl.cap_chunks;
+    pl.lp.C = pl.C;
+    pl.lp.B = B;
+    pl.lp.cmagic = chunk_magic(pl.C);
+    return ALLS_OK;
+}
 
-        HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
-        launch_fill_i32(lp.hdr, n_nodes * ALLS_DIR, 0, s);
-        launch_fill_i32(lp.misc, MISC_SIZE, 0, s);
-        HIP_TRY(hipEventRecord(ws.ev[1], s), "event");
-        if (mi.kind == 3) launch_insert_hex20(dm, lp, mi.f32 ? 1 : 0, s);
-        else launch_insert_2d(dm, lp, mi.kind, o.block_threads, s);
-        HIP_TRY(hipGetLastError(), "launch insert");
-        HIP_TRY(hipEventRecord(ws.ev[2], s), "event");
-        launch_count(lp, G, mi.f32 ? 1 : 0, ws.u.as<int32_t>(), ws.big.as<int32_t>(), s);
-        HIP_TRY(hipGetLastError(), "launch count");
-        HIP_TRY(hipEventRecord(ws.ev[3], s), "event");
-        launch_scan_total(ws.u.as<int32_t>(), n_nodes, ws.tiles.as<int64_t>(), ntiles, s);
-        HIP_TRY(hipGetLastError(), "launch scan");
-        HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
-        HIP_TRY(hipMemcpyAsync(ws.host_misc, lp.misc, MISC_SIZE * sizeof(int32_t), hipMemcpyDeviceToHost, s),
-                "read flags");
-        HIP_TRY(hipMemcpyAsync(ws.host_misc + 32, ws.tiles.as<int64_t>() + ntiles, sizeof(int64_t),
-                               hipMemcpyDeviceToHost, s), "read total");
-        HIP_TRY(hipStreamSynchronize(s), "sync after scan");
-        const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
-        if (f[MISC_ERR + ERR_INVALID_MESH]) {
-            if (err) *err = "mesh has an element with a node id out of range or a wrong element type";
-            return ALLS_E_INVALID;
-        }
-        if (f[MISC_ERR + ERR_SPIN_TIMEOUT]) {
-            if (err) *err = "list chunk publication timed out";
-            return ALLS_E_HIP;
-        }
-        const int32_t need = f[MISC_ERR + ERR_DIR_OVERFLOW];
-        if (need > 0 || f[MISC_ERR + ERR_POOL_EXHAUSTED] || f[MISC_ERR + ERR_ROW_TOO_BIG]) {
-            if (++retries > 6) {
-                if (err) *err = "list directory overflow after retries";
-                return ALLS_E_POOL_OVERFLOW;
-            }
-            if (need > 0 || f[MISC_ERR + ERR_ROW_TOO_BIG]) {
-                int64_t nc = 2 * (int64_t)C;
-                while (nc * ALLS_DIR < need) nc *= 2;
-                if (nc > ALLS_MAX_CHUNK) {
-                    if (err)
-                        *err = "a node row needs " + std::to_string(need) + " list entries; at most " +
-                               std::to_string(long_row_limit()) + " are supported";
-                    return ALLS_E_UNSUPPORTED;
-                }
-                C = (int)nc;
-            } else {
-                full_pool = true;
-            }
-            continue;
-        }
-        max_big = f[MISC_MAX_BIG];
-        if (max_big > long_row_limit()) {
-            if (err)
-                *err = "a node row has " + std::to_string(max_big) + " list entries; at most " +
-                       std::to_string(long_row_limit()) + " are supported";
-            return ALLS_E_UNSUPPORTED;
-        }
-        total = ws.host_misc[32];
-        if (st) {
-            st->big_rows = f[MISC_BIG_COUNT];
-            st->pool_chunks_used = n_nodes + f[MISC_OVF_TOP];
-            st->pool_chunks_cap = cap_chunks;
-        }
-        break;
+// Header reset, integration + insertion, count, scan totals, and the
+// read-back of the flags and the total into pinned memory.  `events`
+// records the stage events (not under graph capture).
+static alls_status enqueue_front(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o,
+                                 const Pipeline& pl, hipStream_t s, bool events, std::string* err) {
+    const int64_t n_nodes = dm.n_nodes;
+    const int64_t maxb = insert_max_blocks(o.grid_per_cu);
+    if (events) HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
+    launch_fill_i32(pl.lp.hdr, n_nodes * ALLS_DIR, 0, s);
+    launch_fill_i32(pl.lp.misc, MISC_SIZE, 0, s);
+    if (events) HIP_TRY(hipEventRecord(ws.ev[1], s), "event");
+    if (mi.kind == 3) launch_insert_hex20(dm, pl.lp, mi.f32 ? 1 : 0, maxb, s);
+    else launch_insert_2d(dm, pl.lp, mi.kind, o.block_threads, maxb, s);
+    HIP_TRY(hipGetLastError(), "launch insert");
+    if (events) HIP_TRY(hipEventRecord(ws.ev[2], s), "event");
+    launch_count(pl.lp, pl.G, mi.f32 ? 1 : 0, ws.u.as<int32_t>(), ws.big.as<int32_t>(), s);
+    HIP_TRY(hipGetLastError(), "launch count");
+    if (events) HIP_TRY(hipEventRecord(ws.ev[3], s), "event");
+    launch_scan_total(ws.u.as<int32_t>(), n_nodes, ws.tiles.as<int64_t>(), pl.ntiles, s);
+    HIP_TRY(hipGetLastError(), "launch scan");
+    if (events) HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
+    HIP_TRY(hipMemcpyAsync(ws.host_misc, pl.lp.misc, MISC_SIZE * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+            "read flags");
+    HIP_TRY(hipMemcpyAsync(ws.host_misc + 32, ws.tiles.as<int64_t>() + pl.ntiles, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, s), "read total");
+    return ALLS_OK;
+}
+
+// Reads the flags after the stream has synchronised.  *retry: run the front
+// half again with pl updated (larger chunks or the full pool).
+static alls_status check_front(Workspace& ws, Pipeline& pl, bool* retry, std::string* err) {
+    *retry = false;
+    const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
+    if (f[MISC_ERR + ERR_INVALID_MESH]) {
+        if (err) *err = "mesh has an element with a node id out of range or a wrong element type";
+        return ALLS_E_INVALID;
     }
-    const int64_t nnz = total * dpn * dpn;
+    if (f[MISC_ERR + ERR_SPIN_TIMEOUT]) {
+        if (err) *err = "list chunk publication timed out";
+        return ALLS_E_HIP;
+    }
+    const int32_t need = f[MISC_ERR + ERR_DIR_OVERFLOW];
+    if (need > 0 || f[MISC_ERR + ERR_POOL_EXHAUSTED] || f[MISC_ERR + ERR_ROW_TOO_BIG]) {
+        if (++pl.retries > 6) {
+            if (err) *err = "list directory overflow after retries";
+            return ALLS_E_POOL_OVERFLOW;
+        }
+        if (need > 0 || f[MISC_ERR + ERR_ROW_TOO_BIG]) {
+            int64_t nc = 2 * (int64_t)pl.C;
+            while (nc * ALLS_DIR < need) nc *= 2;
+            if (nc > ALLS_MAX_CHUNK) {
+                if (err)
+                    *err = "a node row needs " + std::to_string(need) + " list entries; at most " +
+                           std::to_string(long_row_limit()) + " are supported";
+                return ALLS_E_UNSUPPORTED;
+            }
+            pl.C = (int)nc;
+        } else {
+            pl.full_pool = true;
+        }
+        *retry = true;
+        return ALLS_OK;
+    }
+    pl.max_big = f[MISC_MAX_BIG];
+    if (pl.max_big > long_row_limit()) {
+        if (err)
+            *err = "a node row has " + std::to_string(pl.max_big) + " list entries; at most " +
+                   std::to_string(long_row_limit()) + " are supported";
+        return ALLS_E_UNSUPPORTED;
+    }
+    pl.total = ws.host_misc[32];
+    return ALLS_OK;
+}
+
+// row_ptr from the scan, then the per-row merge and write.
+static alls_status enqueue_back(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const Pipeline& pl,
+                                alls_csr* out, hipStream_t s, bool events, std::string* err) {
+    if (events) HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
+    launch_scan_apply(ws.u.as<int32_t>(), dm.n_nodes, ws.tiles.as<int64_t>(), mi.dpn, out->row_ptr,
+                      out->row_ptr_is_i64, s);
+    HIP_TRY(hipGetLastError(), "launch scan apply");
+    launch_write(pl.lp, pl.G, mi.f32 ? 1 : 0, mi.dpn, pl.max_big, ws.big.as<int32_t>(), out->row_ptr,
+                 out->row_ptr_is_i64, out->col_ind, out->val, s);
+    HIP_TRY(hipGetLastError(), "launch write");
+    if (events) HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+    return ALLS_OK;
+}
+
+namespace {
+uint64_t mix(uint64_t h, uint64_t v) { return h ^ (v + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2)); }
+uint64_t mixp(uint64_t h, const void* p) { return mix(h, (uint64_t)(uintptr_t)p); }
+uint64_t mixd(uint64_t h, double d) {
+    uint64_t b;
+    std::memcpy(&b, &d, sizeof b);
+    return mix(h, b);
+}
+
+// Every kernel argument of the front half: a captured graph replays them.
+uint64_t front_key(const Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o,
+                   const Pipeline& pl) {
+    uint64_t h = 0x5eed;
+    for (const void* p : {(const void*)dm.coords, (const void*)dm.conn, (const void*)dm.types,
+                          (const void*)dm.E_elem, (const void*)pl.lp.hdr, (const void*)pl.lp.keys, (const void*)pl.lp.vals,
+                          (const void*)pl.lp.misc, (const void*)ws.u.p, (const void*)ws.big.p, (const void*)ws.tiles.p,
+                          (const void*)ws.host_misc})
+        h = mixp(h, p);
+    for (int64_t v : {dm.n_nodes, dm.n_elems, (int64_t)dm.npe, (int64_t)dm.dim, (int64_t)dm.uniform_type,
+                      (int64_t)mi.kind, (int64_t)mi.f32, (int64_t)pl.C, pl.cap_chunks, pl.ntiles, (int64_t)pl.G,
+                      (int64_t)o.block_threads, (int64_t)o.grid_per_cu})
+        h = mix(h, (uint64_t)v);
+    return mixd(mixd(mixd(h, dm.E), dm.nu), dm.thickness);
+}
+
+uint64_t back_key(const Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const Pipeline& pl,
+                  const alls_csr* out) {
+    uint64_t h = 0xbac;
+    for (const void* p : {(const void*)pl.lp.hdr, (const void*)pl.lp.keys, (const void*)pl.lp.vals, (const void*)pl.lp.misc,
+                          (const void*)ws.u.p, (const void*)ws.big.p, (const void*)ws.tiles.p, (const void*)out->row_ptr,
+                          (const void*)out->col_ind, (const void*)out->val})
+        h = mixp(h, p);
+    for (int64_t v : {dm.n_nodes, (int64_t)mi.dpn, (int64_t)mi.f32, (int64_t)pl.C, pl.cap_chunks, pl.ntiles,
+                      (int64_t)pl.G, (int64_t)pl.max_big, (int64_t)out->row_ptr_is_i64})
+        h = mix(h, (uint64_t)v);
+    return h;
+}
+}  // namespace
+
+void GraphSlot::reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key = 0;
+}
+
+// Launches the graph of `enqueue` on s, capturing it first when the key
+// (every argument the captured kernels hold) changed.
+template <class F>
+static alls_status graph_launch(GraphSlot& g, uint64_t key, hipStream_t s, F&& enqueue, std::string* err) {
+    if (!g.exec || g.key != key) {
+        g.reset();
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
+        const alls_status st = enqueue();
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (st != ALLS_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        HIP_TRY(e, "end graph capture");
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+            g.exec = nullptr;
+            return hip_fail(ei, "instantiate graph", err);
+        }
+        g.key = key;
+    }
+    HIP_TRY(hipGraphLaunch(g.exec, s), "graph launch");
+    return ALLS_OK;
+}
+
+alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
+                     hipStream_t s, alls_stats* st, std::string* err, bool use_graph) {
+    HIP_TRY(ws.init(), "workspace init");
+    const int dpn = mi.dpn;
+    const int64_t n_nodes = dm.n_nodes;
+    const int64_t n_rows = n_nodes * dpn;
+    Pipeline pl;
+    pl.C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
+    pl.C = std::min(std::max(pl.C, 4), ALLS_MAX_CHUNK);
+    pl.G = small_group(mi.kind);
+    pl.ntiles = scan_tile_count(n_nodes);
+    // Graph replays do not record the per-stage events; the whole front and
+    // back halves are timed around the launches instead.
+    const bool stage_events = !use_graph;
+    for (;;) {
+        alls_status rs = alloc_lists(ws, dm, mi, pl, err);
+        if (rs != ALLS_OK) return rs;
+        if (use_graph) {
+            HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
+            rs = graph_launch(ws.g_front, front_key(ws, dm, mi, o, pl), s,
+                              [&] { return enqueue_front(ws, dm, mi, o, pl, s, false, err); }, err);
+            if (rs != ALLS_OK) return rs;
+            HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
+        } else {
+            rs = enqueue_front(ws, dm, mi, o, pl, s, stage_events, err);
+            if (rs != ALLS_OK) return rs;
+        }
+        HIP_TRY(hipStreamSynchronize(s), "sync after scan");
+        bool retry = false;
+        rs = check_front(ws, pl, &retry, err);
+        if (rs != ALLS_OK) return rs;
+        if (!retry) break;
+    }
+    const int64_t nnz = pl.total * dpn * dpn;
     const int i64 = (o.row_ptr_i64 || nnz > (int64_t)INT32_MAX) ? 1 : 0;
     alls_status rs = ensure_csr(out, n_rows, nnz, i64);
     if (rs != ALLS_OK) {
         if (err) *err = "cannot allocate output CSR";
         return rs;
     }
-    HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
-    launch_scan_apply(ws.u.as<int32_t>(), n_nodes, ws.tiles.as<int64_t>(), dpn, out->row_ptr, i64, s);
-    HIP_TRY(hipGetLastError(), "launch scan apply");
-    launch_write(lp, G, mi.f32 ? 1 : 0, dpn, max_big, ws.big.as<int32_t>(), out->row_ptr, i64, out->col_ind, out->val, s);
-    HIP_TRY(hipGetLastError(), "launch write");
-    HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+    if (use_graph) {
+        HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
+        rs = graph_launch(ws.g_back, back_key(ws, dm, mi, pl, out), s,
+                          [&] { return enqueue_back(ws, dm, mi, pl, out, s, false, err); }, err);
+        if (rs != ALLS_OK) return rs;
+        HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+    } else {
+        rs = enqueue_back(ws, dm, mi, pl, out, s, stage_events, err);
+        if (rs != ALLS_OK) return rs;
+    }
     out->pattern_id = 0;
     HIP_TRY(hipStreamSynchronize(s), "sync after write");
     if (st) {
-        st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
-        st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
-        st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
-        st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]);
-        st->ms_write = ev_ms(ws.ev[5], ws.ev[6]);   // scan apply + sort/merge/write
-        st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);
+        *st = alls_stats{};
+        if (stage_events) {
+            st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
+            st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
+            st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
+            st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]);
+        }
+        st->ms_write = ev_ms(ws.ev[5], ws.ev[6]);   // scan apply + merge/write
+        st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);   // includes the flag read-back between the halves
         st->n_elems = dm.n_elems;
         st->n_rows = n_rows;
         st->nnz = nnz;
         st->list_entries = mi.entries_bound;
-        st->chunk_slots = C;
-        st->retries = retries;
+        st->chunk_slots = pl.C;
+        st->retries = pl.retries;
         st->scratch_bytes = (int64_t)(ws.hdr.bytes + ws.keys.bytes + ws.vals.bytes);
+        const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
+        st->big_rows = f[MISC_BIG_COUNT];
+        st->pool_chunks_used = n_nodes + f[MISC_OVF_TOP];
+        st->pool_chunks_cap = pl.cap_chunks;
     }
     return ALLS_OK;
 }
@@ -632,6 +814,8 @@ alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
     if (s != ALLS_OK) return s;
     DevMesh dm;
     s = stage_mesh(ctx->ws, mesh, mi, &dm, ctx->stream, &ctx->err);
+    if (s != ALLS_OK) return s;
+    s = apply_tuning(ctx, dm, mi, &o);
     if (s != ALLS_OK) return s;
     alls_stats st{};
     s = run_alls(ctx->ws, dm, mi, o, out, ctx->stream, &st, &ctx->err);

@@ -788,9 +788,8 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
                        hipStream_t s) {
     const int64_t ntiles = scan_tile_count(n);
     if (ntiles == 0) {
-        // No nodes: row_ptr = {0}.
-        if (i64) hipMemsetAsync(row_ptr, 0, sizeof(int64_t), s);
-        else hipMemsetAsync(row_ptr, 0, sizeof(int32_t), s);
+        // No nodes: row_ptr = {0} (a fill kernel, so that graphs may hold it).
+        launch_fill_i32(static_cast<int32_t*>(row_ptr), i64 ? 2 : 1, 0, s);
         return;
     }
     if (i64)

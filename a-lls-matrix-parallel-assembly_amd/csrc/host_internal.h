@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 
 #include "alls.h"
@@ -48,19 +49,30 @@ struct MeshInfo {
     bool f32 = false;          // FP32-MFMA Hex20 integration
 };
 
+// An instantiated hipGraph and the key of the arguments it was captured with.
+struct GraphSlot {
+    uint64_t key = 0;
+    hipGraphExec_t exec = nullptr;
+    void reset();
+};
+
 // Per-stream scratch: list headers, pool, consolidation arrays and a device
 // copy of host meshes.  A context owns one; the server owns one per slot.
 struct Workspace {
     DevBuf hdr, keys, vals, misc, u, big, tiles;
     DevBuf coords, conn, types, E;
+    DevBuf bdesc;                   // batch: source-mesh table, boundary indices and offsets
     int64_t* host_misc = nullptr;   // pinned readback: misc ints + scan total
     void* pinned = nullptr;         // pinned staging of host inputs (grow-only)
     size_t pinned_bytes = 0;
     // Copies host arrays to device through the pinned buffer (one async copy
     // each); pageable copies would block behind other streams' work.
     hipError_t upload(void* const* dst, const void* const* src, const size_t* bytes, int n, hipStream_t s);
+    // The pinned buffer, grown to `bytes`, for callers that fill it directly.
+    hipError_t staging(size_t bytes, char** host);
     hipEvent_t ev[8] = {};
     bool events = false;
+    GraphSlot g_front, g_back;      // captured halves of the pipeline (batch path)
     hipError_t init();
     void destroy();
     size_t device_bytes() const;
@@ -72,8 +84,11 @@ alls_status classify_mesh(const alls_mesh* m, const alls_opts& o, MeshInfo* mi, 
 alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, DevMesh* dm, hipStream_t s,
                        std::string* err);
 // Full A-LLS pipeline on `s`: insert, count, scan, write.  Synchronises `s`.
+// use_graph: the halves before and after the one host read-back run as two
+// hipGraphs, captured on first use and replayed while their arguments (buffer
+// addresses, sizes, material) stay the same.
 alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
-                     hipStream_t s, alls_stats* st, std::string* err);
+                     hipStream_t s, alls_stats* st, std::string* err, bool use_graph = false);
 // Makes sure `out` can hold n_rows rows and nnz entries.
 alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64);
 void free_csr(alls_csr* out);
@@ -89,6 +104,12 @@ int64_t pattern_bytes(const alls_pattern* p);
 
 }  // namespace alls
 
+// Autotuned launch settings of one (element kind, precision, size class).
+struct alls_tuned {
+    int32_t block_threads, chunk_slots, grid_per_cu;
+    double ms;
+};
+
 struct alls_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -96,4 +117,11 @@ struct alls_ctx {
     std::string err;
     alls_stats stats{};
     alls::Workspace ws;
+    std::map<uint64_t, alls_tuned> tuned;   // autotuner results by tune_key()
 };
+
+namespace alls {
+// opts.autotune: copies the tuned block_threads / chunk_slots / grid_per_cu
+// of the mesh's kind and size class into *o, tuning on dm first when new.
+alls_status apply_tuning(alls_ctx* ctx, const DevMesh& dm, const MeshInfo& mi, alls_opts* o);
+}  // namespace alls

@@ -29,9 +29,7 @@ __device__ __forceinline__ void store_entry(const ListPool& p, int64_t idx, int3
 
 // MODE 0: all Q4, 1: all T3, 2: mixed (per-element type array).
 template <int MODE>
-__global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
-    const int64_t e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (e >= m.n_elems) return;
+__device__ __forceinline__ void insert_element(const DevMesh& m, const ListPool& p, int64_t e) {
     Elem2D el;
     if (!el.load<MODE>(m, e, p.misc)) return;
     const int nn = el.nn;
@@ -61,6 +59,17 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
             store_entry(p, ch * p.C + (s - chunk_of(p, s) * p.C), el.nd[j], (int32_t)e, k);
         }
     }
+}
+
+// Grid-stride over the elements: one launch covers any number of elements
+// with a grid sized to the device (persistent) or one thread per element.
+// Lanes of a wave take the same iteration together, and a chunk is always
+// published in the same iteration that reserved it, before any wait.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; e < m.n_elems; e += stride)
+        insert_element<MODE>(m, p, e);
 }
 
 __device__ __forceinline__ void add_val(double* v, double x, bool atomic) {
@@ -220,10 +229,11 @@ void launch_fill_f64(double* p, int64_t n, double v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
 }
 
-void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, hipStream_t s) {
+void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, int64_t max_blocks, hipStream_t s) {
     if (m.n_elems <= 0) return;
     if (block != 64 && block != 128 && block != 256) block = 256;
-    const unsigned g = grid_for(m.n_elems, block);
+    unsigned g = grid_for(m.n_elems, block);
+    if (max_blocks > 0 && g > max_blocks) g = (unsigned)max_blocks;
     if (mode == 0) hipLaunchKernelGGL(k_insert_2d<0>, dim3(g), dim3(block), 0, s, m, p);
     else if (mode == 1) hipLaunchKernelGGL(k_insert_2d<1>, dim3(g), dim3(block), 0, s, m, p);
     else hipLaunchKernelGGL(k_insert_2d<2>, dim3(g), dim3(block), 0, s, m, p);

@@ -83,7 +83,7 @@ template <> struct OutRow<float> {
 // pattern; 2: plain "+=" scatter (one colour); 3: write each block to its
 // precomputed staging slot (value-only re-assembly, no atomics).
 template <typename T, int EMIT>
-__global__ __launch_bounds__(64) void k_hex20(DevMesh m, ListPool p, Scatter sc, int32_t* misc) {
+__global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter sc, int32_t* misc) {
     constexpr int LDS_T = (2 * KS * XS > 60 * MS) ? 2 * KS * XS : 60 * MS;
     __shared__ T lds[LDS_T];
     __shared__ double xyz[60];
@@ -91,177 +91,181 @@ __global__ __launch_bounds__(64) void k_hex20(DevMesh m, ListPool p, Scatter sc,
     __shared__ int32_t posv[20];
     __shared__ int32_t bad;
 
-    const int64_t idx = xcd_block(blockIdx.x, gridDim.x);
-    if (idx >= (EMIT == 0 ? m.n_elems : sc.count)) return;
-    const int64_t e = (EMIT != 0 && sc.elist) ? (int64_t)sc.elist[idx] : idx;
     const int lane = threadIdx.x;
-    if (lane == 0) bad = 0;
-    __syncthreads();
-    if (lane < 20) {
-        const int32_t n = m.conn[e * m.npe + lane];
-        if (n < 0 || n >= m.n_nodes) {
-            bad = 1;
-            nodes[lane] = 0;
-        } else {
-            nodes[lane] = n;
-            xyz[lane * 3 + 0] = m.coords[3 * (int64_t)n + 0];
-            xyz[lane * 3 + 1] = m.coords[3 * (int64_t)n + 1];
-            xyz[lane * 3 + 2] = m.coords[3 * (int64_t)n + 2];
-        }
-    }
-    __syncthreads();
-    if (bad || (m.types && m.types[e] != 2)) {
-        if (lane == 0) st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
-        return;
-    }
-
-    // Gauss-point gradients: lane g < 27 fills row g of X and W*X.
-    T* X = lds;
-    T* Y = lds + KS * XS;
-    if (lane < KS) {
-        if (lane < 27) {
-            const int gi = lane / 9, gj = (lane / 3) % 3, gk = lane % 3;
-            const double xi = HGP[gi], eta = HGP[gj], zeta = HGP[gk];
-            const double weight = HGW[gi] * HGW[gj] * HGW[gk];
-            double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int n = 0; n < 20; ++n) {
-                double dx, de, dz;
-                hex20_dshape(n, xi, eta, zeta, dx, de, dz);
-                const double px = xyz[3 * n], py = xyz[3 * n + 1], pz = xyz[3 * n + 2];
-                J[0] += dx * px; J[1] += dx * py; J[2] += dx * pz;
-                J[3] += de * px; J[4] += de * py; J[5] += de * pz;
-                J[6] += dz * px; J[7] += dz * py; J[8] += dz * pz;
-            }
-            double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
-                         J[2] * (J[3] * J[7] - J[4] * J[6]);
-            if (fabs(det) < 1e-15) det = 1e-15;  // Hex:156
-            const double r = 1.0 / det;
-            const double i0 = (J[4] * J[8] - J[5] * J[7]) * r, i1 = (J[2] * J[7] - J[1] * J[8]) * r;
-            const double i2 = (J[1] * J[5] - J[2] * J[4]) * r, i3 = (J[5] * J[6] - J[3] * J[8]) * r;
-            const double i4 = (J[0] * J[8] - J[2] * J[6]) * r, i5 = (J[2] * J[3] - J[0] * J[5]) * r;
-            const double i6 = (J[3] * J[7] - J[4] * J[6]) * r, i7 = (J[1] * J[6] - J[0] * J[7]) * r;
-            const double i8 = (J[0] * J[4] - J[1] * J[3]) * r;
-            const double w = weight * det;
-            T* xr = X + lane * XS;
-            T* yr = Y + lane * XS;
-            for (int n = 0; n < 20; ++n) {
-                double dx, de, dz;
-                hex20_dshape(n, xi, eta, zeta, dx, de, dz);
-                const double gx = i0 * dx + i1 * de + i2 * dz;
-                const double gy = i3 * dx + i4 * de + i5 * dz;
-                const double gz = i6 * dx + i7 * de + i8 * dz;
-                xr[3 * n + 0] = (T)gx; xr[3 * n + 1] = (T)gy; xr[3 * n + 2] = (T)gz;
-                yr[3 * n + 0] = (T)(w * gx); yr[3 * n + 1] = (T)(w * gy); yr[3 * n + 2] = (T)(w * gz);
-            }
-            for (int c = 60; c < XS; ++c) xr[c] = yr[c] = (T)0;
-        } else {
-            for (int c = 0; c < XS; ++c) X[lane * XS + c] = Y[lane * XS + c] = (T)0;
-        }
-    }
-    __syncthreads();
-
-    // M = X^T Y on the matrix cores: tiles (tr <= tc) of 16x16, A = X^T, B = Y.
-    typedef typename Acc<T>::type acc_t;
-    acc_t acc[10];
-#pragma unroll
-    for (int t = 0; t < 10; ++t) acc[t] = acc_t{0, 0, 0, 0};
-    const int li = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int s = 0; s < KS / 4; ++s) {
-        T av[4], bv[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            av[t] = X[(4 * s + lk) * XS + 16 * t + li];
-            bv[t] = Y[(4 * s + lk) * XS + 16 * t + li];
-        }
-        int t = 0;
-#pragma unroll
-        for (int tr = 0; tr < 4; ++tr)
-#pragma unroll
-            for (int tc = tr; tc < 4; ++tc) {
-                acc[t] = mfma(av[tr], bv[tc], acc[t]);
-                ++t;
-            }
-    }
-    __syncthreads();   // X/Y are overwritten by M below
-
-    // Scatter the tiles into M (row stride MS), mirroring the upper triangle.
-    T* M = lds;
-    {
-        int t = 0;
-#pragma unroll
-        for (int tr = 0; tr < 4; ++tr)
-#pragma unroll
-            for (int tc = tr; tc < 4; ++tc) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = 16 * tr + OutRow<T>::row(lk, q), c = 16 * tc + li;
-                    if (r < 60 && c < 60) {
-                        M[r * MS + c] = acc[t][q];
-                        if (tr != tc) M[c * MS + r] = acc[t][q];
-                    }
-                }
-                ++t;
-            }
-    }
-    const double E = m.E_elem ? m.E_elem[e] : m.E;
-    const double nu = m.nu;
-    const T lam = (T)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
-    const T mu = (T)(E / (2.0 * (1.0 + nu)));
-    if (EMIT == 0) {
-        // Reserve 20 slots in each of the element's 20 node rows.
+    const int64_t count = EMIT == 0 ? m.n_elems : sc.count;
+    // Grid-stride over the elements (a grid sized to the device, or one block per element).
+    for (int64_t idx = xcd_block(blockIdx.x, gridDim.x); idx < count; idx += gridDim.x) {
+        const int64_t e = (EMIT != 0 && sc.elist) ? (int64_t)sc.elist[idx] : idx;
+        if (lane == 0) bad = 0;
+        __syncthreads();
         if (lane < 20) {
-            const int32_t pos = list_reserve(p, nodes[lane], 20);
-            list_own_chunks(p, nodes[lane], pos, 20);
-            posv[lane] = pos;
+            const int32_t n = m.conn[e * m.npe + lane];
+            if (n < 0 || n >= m.n_nodes) {
+                bad = 1;
+                nodes[lane] = 0;
+            } else {
+                nodes[lane] = n;
+                xyz[lane * 3 + 0] = m.coords[3 * (int64_t)n + 0];
+                xyz[lane * 3 + 1] = m.coords[3 * (int64_t)n + 1];
+                xyz[lane * 3 + 2] = m.coords[3 * (int64_t)n + 2];
+            }
         }
-    }
-    __syncthreads();
+        __syncthreads();
+        if (bad || (m.types && m.types[e] != 2)) {
+            if (lane == 0) st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
+            __syncthreads();
+            continue;
+        }
 
-    for (int pr = lane; pr < 400; pr += 64) {
-        const int I = pr / 20, Jn = pr % 20;
-        T mb[3][3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) mb[r][c] = M[(3 * I + r) * MS + 3 * Jn + c];
-        const T tr = mb[0][0] + mb[1][1] + mb[2][2];
-        T kb[3][3];
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) kb[r][c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
-        if (EMIT == 0) {
-            const int32_t s = posv[I] + Jn;
-            const int64_t ch = list_chunk(p, nodes[I], s);
-            if (ch < 0) continue;
-            const int64_t id = ch * p.C + (s - chunk_of(p, s) * p.C);
-            T* dst = static_cast<T*>(p.vals) + id * 9;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
-            p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
-        } else if (EMIT == 3) {
-            T* dst = static_cast<T*>(sc.stage) + sc.slot[e * 400 + pr] * 9;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
-        } else {
-            const int64_t a = nodes[I];
-            const int64_t rs = rp_at(sc.row_ptr, sc.i64, 3 * a);
-            const int64_t rowlen = rp_at(sc.row_ptr, sc.i64, 3 * a + 1) - rs;
-            double* v = sc.val + rs + 3 * (int64_t)sc.map[e * 400 + pr];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    double* q = v + r * rowlen + c;
-                    if (EMIT == 1) atomicAdd(q, (double)kb[r][c]);
-                    else *q += (double)kb[r][c];
+        // Gauss-point gradients: lane g < 27 fills row g of X and W*X.
+        T* X = lds;
+        T* Y = lds + KS * XS;
+        if (lane < KS) {
+            if (lane < 27) {
+                const int gi = lane / 9, gj = (lane / 3) % 3, gk = lane % 3;
+                const double xi = HGP[gi], eta = HGP[gj], zeta = HGP[gk];
+                const double weight = HGW[gi] * HGW[gj] * HGW[gk];
+                double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int n = 0; n < 20; ++n) {
+                    double dx, de, dz;
+                    hex20_dshape(n, xi, eta, zeta, dx, de, dz);
+                    const double px = xyz[3 * n], py = xyz[3 * n + 1], pz = xyz[3 * n + 2];
+                    J[0] += dx * px; J[1] += dx * py; J[2] += dx * pz;
+                    J[3] += de * px; J[4] += de * py; J[5] += de * pz;
+                    J[6] += dz * px; J[7] += dz * py; J[8] += dz * pz;
+                }
+                double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                             J[2] * (J[3] * J[7] - J[4] * J[6]);
+                if (fabs(det) < 1e-15) det = 1e-15;  // Hex:156
+                const double r = 1.0 / det;
+                const double i0 = (J[4] * J[8] - J[5] * J[7]) * r, i1 = (J[2] * J[7] - J[1] * J[8]) * r;
+                const double i2 = (J[1] * J[5] - J[2] * J[4]) * r, i3 = (J[5] * J[6] - J[3] * J[8]) * r;
+                const double i4 = (J[0] * J[8] - J[2] * J[6]) * r, i5 = (J[2] * J[3] - J[0] * J[5]) * r;
+                const double i6 = (J[3] * J[7] - J[4] * J[6]) * r, i7 = (J[1] * J[6] - J[0] * J[7]) * r;
+                const double i8 = (J[0] * J[4] - J[1] * J[3]) * r;
+                const double w = weight * det;
+                T* xr = X + lane * XS;
+                T* yr = Y + lane * XS;
+                for (int n = 0; n < 20; ++n) {
+                    double dx, de, dz;
+                    hex20_dshape(n, xi, eta, zeta, dx, de, dz);
+                    const double gx = i0 * dx + i1 * de + i2 * dz;
+                    const double gy = i3 * dx + i4 * de + i5 * dz;
+                    const double gz = i6 * dx + i7 * de + i8 * dz;
+                    xr[3 * n + 0] = (T)gx; xr[3 * n + 1] = (T)gy; xr[3 * n + 2] = (T)gz;
+                    yr[3 * n + 0] = (T)(w * gx); yr[3 * n + 1] = (T)(w * gy); yr[3 * n + 2] = (T)(w * gz);
+                }
+                for (int c = 60; c < XS; ++c) xr[c] = yr[c] = (T)0;
+            } else {
+                for (int c = 0; c < XS; ++c) X[lane * XS + c] = Y[lane * XS + c] = (T)0;
+            }
+        }
+        __syncthreads();
+
+        // M = X^T Y on the matrix cores: tiles (tr <= tc) of 16x16, A = X^T, B = Y.
+        typedef typename Acc<T>::type acc_t;
+        acc_t acc[10];
+    #pragma unroll
+        for (int t = 0; t < 10; ++t) acc[t] = acc_t{0, 0, 0, 0};
+        const int li = lane & 15, lk = lane >> 4;
+    #pragma unroll
+        for (int s = 0; s < KS / 4; ++s) {
+            T av[4], bv[4];
+    #pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                av[t] = X[(4 * s + lk) * XS + 16 * t + li];
+                bv[t] = Y[(4 * s + lk) * XS + 16 * t + li];
+            }
+            int t = 0;
+    #pragma unroll
+            for (int tr = 0; tr < 4; ++tr)
+    #pragma unroll
+                for (int tc = tr; tc < 4; ++tc) {
+                    acc[t] = mfma(av[tr], bv[tc], acc[t]);
+                    ++t;
                 }
         }
+        __syncthreads();   // X/Y are overwritten by M below
+
+        // Scatter the tiles into M (row stride MS), mirroring the upper triangle.
+        T* M = lds;
+        {
+            int t = 0;
+    #pragma unroll
+            for (int tr = 0; tr < 4; ++tr)
+    #pragma unroll
+                for (int tc = tr; tc < 4; ++tc) {
+    #pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int r = 16 * tr + OutRow<T>::row(lk, q), c = 16 * tc + li;
+                        if (r < 60 && c < 60) {
+                            M[r * MS + c] = acc[t][q];
+                            if (tr != tc) M[c * MS + r] = acc[t][q];
+                        }
+                    }
+                    ++t;
+                }
+        }
+        const double E = m.E_elem ? m.E_elem[e] : m.E;
+        const double nu = m.nu;
+        const T lam = (T)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+        const T mu = (T)(E / (2.0 * (1.0 + nu)));
+        if (EMIT == 0) {
+            // Reserve 20 slots in each of the element's 20 node rows.
+            if (lane < 20) {
+                const int32_t pos = list_reserve(p, nodes[lane], 20);
+                list_own_chunks(p, nodes[lane], pos, 20);
+                posv[lane] = pos;
+            }
+        }
+        __syncthreads();
+
+        for (int pr = lane; pr < 400; pr += 64) {
+            const int I = pr / 20, Jn = pr % 20;
+            T mb[3][3];
+    #pragma unroll
+            for (int r = 0; r < 3; ++r)
+    #pragma unroll
+                for (int c = 0; c < 3; ++c) mb[r][c] = M[(3 * I + r) * MS + 3 * Jn + c];
+            const T tr = mb[0][0] + mb[1][1] + mb[2][2];
+            T kb[3][3];
+    #pragma unroll
+            for (int r = 0; r < 3; ++r)
+    #pragma unroll
+                for (int c = 0; c < 3; ++c) kb[r][c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
+            if (EMIT == 0) {
+                const int32_t s = posv[I] + Jn;
+                const int64_t ch = list_chunk(p, nodes[I], s);
+                if (ch < 0) continue;
+                const int64_t id = ch * p.C + (s - chunk_of(p, s) * p.C);
+                T* dst = static_cast<T*>(p.vals) + id * 9;
+    #pragma unroll
+                for (int r = 0; r < 3; ++r)
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
+                p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
+            } else if (EMIT == 3) {
+                T* dst = static_cast<T*>(sc.stage) + sc.slot[e * 400 + pr] * 9;
+    #pragma unroll
+                for (int r = 0; r < 3; ++r)
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
+            } else {
+                const int64_t a = nodes[I];
+                const int64_t rs = rp_at(sc.row_ptr, sc.i64, 3 * a);
+                const int64_t rowlen = rp_at(sc.row_ptr, sc.i64, 3 * a + 1) - rs;
+                double* v = sc.val + rs + 3 * (int64_t)sc.map[e * 400 + pr];
+    #pragma unroll
+                for (int r = 0; r < 3; ++r)
+    #pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        double* q = v + r * rowlen + c;
+                        if (EMIT == 1) atomicAdd(q, (double)kb[r][c]);
+                        else *q += (double)kb[r][c];
+                    }
+            }
+        }
+        __syncthreads();   // LDS is refilled by the next element
     }
 }
 
@@ -269,13 +273,15 @@ __global__ __launch_bounds__(64) void k_hex20(DevMesh m, ListPool p, Scatter sc,
 
 namespace alls {
 
-void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, hipStream_t s) {
+void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, int64_t max_blocks, hipStream_t s) {
     if (m.n_elems <= 0) return;
     const Scatter none{};
+    int64_t g = m.n_elems;
+    if (max_blocks > 0 && g > max_blocks) g = max_blocks;
     if (f32)
-        hipLaunchKernelGGL((k_hex20<float, 0>), dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p, none, p.misc);
+        hipLaunchKernelGGL((k_hex20<float, 0>), dim3((unsigned)g), dim3(64), 0, s, m, p, none, p.misc);
     else
-        hipLaunchKernelGGL((k_hex20<double, 0>), dim3((unsigned)m.n_elems), dim3(64), 0, s, m, p, none, p.misc);
+        hipLaunchKernelGGL((k_hex20<double, 0>), dim3((unsigned)g), dim3(64), 0, s, m, p, none, p.misc);
 }
 
 void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int mode, int f32, int32_t* misc, hipStream_t s) {

@@ -83,7 +83,11 @@ typedef struct {
     int32_t block_threads;    /* threads per block of the insert kernel; 0 => tuned default */
     int32_t row_ptr_i64;      /* 1 => always int64 row_ptr; otherwise int64 only when nnz >= 2^31 */
     int32_t timing;           /* 1 => time each stage with hipEvents (alls_last_stats) */
-    int32_t reserved[3];
+    int32_t grid_per_cu;      /* insert kernel grid: > 0 => persistent, this many blocks per CU looping
+                                 over the elements; 0 => one thread (2D) or one wave (Hex20) per element */
+    int32_t autotune;         /* 1 => take block_threads / chunk_slots / grid_per_cu from the context's
+                                 autotuner (tuned on first use for the mesh's kind and size class) */
+    int32_t reserved;
 } alls_opts;
 
 /* CSR triple in device memory.  Zero-initialise before first use. */
@@ -188,10 +192,33 @@ alls_status alls_assemble_colored(alls_ctx* ctx, const alls_pattern* pat, const 
                                   const double* E_per_elem, int32_t mem, alls_csr* out);
 
 /* ------------------------------------------------------------ batching --- */
-/* Assembles n meshes of uneven size in one hipGraph-captured launch
- * sequence.  Every stage is a single persistent launch over all meshes. */
-alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshes,
-                                const alls_opts* opts, alls_csr* outs);
+/* Assembles n meshes of one dimension in one pipeline.  The meshes are
+ * packed into one node/element space (mesh k's node ids shifted by the nodes
+ * of meshes 0..k-1), so each stage is ONE launch over all of them whatever
+ * their sizes; the insertion kernel runs persistent (opts->grid_per_cu,
+ * default 8 blocks per CU for 2D and 4 for Hex20); and the stages before and
+ * after the pipeline's one host read-back run as two hipGraphs, captured on
+ * the first call and replayed while the batch keeps its shape (sizes and
+ * material layout; new coordinates or E values replay).
+ * The result is the block-diagonal CSR of the batch: mesh k owns rows
+ * [row_off[k], row_off[k+1]) and nonzeros [nnz_off[k], nnz_off[k+1]), and
+ * its column indices are shifted by row_off[k].  row_off and nnz_off are
+ * caller arrays of n+1 entries (host).  The meshes must share dim, mem, nu
+ * and (2D) thickness; Q4, T3 and mixed 2D meshes may be batched together.
+ * No reference counterpart: each reference program assembles one mesh per
+ * call (Acc:750-838, T3:525, Mix:578, Hex:420). */
+alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshes, const alls_opts* opts,
+                                alls_csr* out, int64_t* row_off, int64_t* nnz_off);
+
+/* Times the pipeline on `mesh` for each candidate launch setting (threads
+ * per block and persistent grid of the insertion kernel, list-pool chunk
+ * size), keeps the fastest for the mesh's element kind, precision and size
+ * class (a power of two of the element count) in the context, and returns
+ * it in *best (the other fields copied from *base).  Calls with
+ * opts->autotune = 1 use the kept setting, tuning on their own mesh the
+ * first time a kind and size class is seen.  Outputs do not depend on the
+ * setting: every column is summed in element order. */
+alls_status alls_autotune(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* base, alls_opts* best);
 
 /* ------------------------------------------------------------- serving --- */
 /* Continuous-batching assembly service on one GPU.  Requests are admitted
