@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liballs.so")
 
 OK, E_INVALID, E_OOM, E_POOL_OVERFLOW, E_HIP, E_BUSY, E_NO_DEVICE, E_UNSUPPORTED = range(8)
+CACHE_COST, CACHE_EAGER = 0, 1
 Q4, T3, HEX20 = 0, 1, 2
 MEM_HOST, MEM_DEVICE = 0, 1
 PREC_F64, PREC_F32_MFMA = 0, 1
@@ -63,7 +64,7 @@ class Stats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("n_streams", C.c_int32), ("hbm_budget_bytes", C.c_int64),
-        ("pattern_cache_entries", C.c_int32), ("reserved", C.c_int32),
+        ("pattern_cache_entries", C.c_int32), ("cache_policy", C.c_int32),
     ]
 
 
@@ -540,10 +541,11 @@ class DeviceMesh:
 class Server:
     """Continuous-batching assembly service (alls_server_*)."""
 
-    def __init__(self, device=0, n_streams=2, hbm_budget_bytes=0, pattern_cache_entries=0):
+    def __init__(self, device=0, n_streams=2, hbm_budget_bytes=0, pattern_cache_entries=0, cache_policy=CACHE_COST):
         cfg = ServerConfig()
         cfg.device, cfg.n_streams = device, n_streams
         cfg.hbm_budget_bytes, cfg.pattern_cache_entries = hbm_budget_bytes, pattern_cache_entries
+        cfg.cache_policy = cache_policy
         h = vp()
         check(lib().alls_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h

@@ -70,7 +70,124 @@ void dev_trim() {
     trim_locked(0);
 }
 
+// ------------------------------------------------------------ device arenas
+// A server reserves its whole HBM budget with one hipMalloc and its worker
+// threads carve every buffer (workspaces, outputs, cached patterns) out of
+// it: best fit over free ranges, coalesced on free.  The serving loop then
+// never calls hipMalloc or hipFree, whose cost grows with the size (hundreds
+// of ms for tens of GB) and which block other threads' HIP calls.
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    std::mutex mu;
+    std::map<size_t, size_t> free_at;          // offset -> length
+    std::multimap<size_t, size_t> free_len;    // length -> offset
+    std::unordered_map<size_t, size_t> live;   // offset -> length
+};
+
+namespace {
+std::mutex g_arena_mu;
+std::vector<Arena*> g_arenas;
+thread_local Arena* t_arena = nullptr;
+
+void arena_insert_free(Arena* a, size_t off, size_t len) {
+    a->free_at[off] = len;
+    a->free_len.emplace(len, off);
+}
+void arena_erase_free(Arena* a, std::map<size_t, size_t>::iterator it) {
+    auto r = a->free_len.equal_range(it->second);
+    for (auto j = r.first; j != r.second; ++j)
+        if (j->second == it->first) {
+            a->free_len.erase(j);
+            break;
+        }
+    a->free_at.erase(it);
+}
+
+void* arena_alloc(Arena* a, size_t bytes) {
+    const size_t need = (bytes + 255) & ~size_t(255);
+    std::lock_guard<std::mutex> lk(a->mu);
+    auto it = a->free_len.lower_bound(need);
+    if (it == a->free_len.end()) return nullptr;
+    const size_t off = it->second, len = it->first;
+    arena_erase_free(a, a->free_at.find(off));
+    if (len > need) arena_insert_free(a, off + need, len - need);
+    a->live[off] = need;
+    return a->base + off;
+}
+
+// True when p belonged to an arena (and is now free in it).
+bool arena_free(void* p) {
+    Arena* a = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_arena_mu);
+        for (Arena* x : g_arenas)
+            if (static_cast<char*>(p) >= x->base && static_cast<char*>(p) < x->base + x->size) a = x;
+    }
+    if (!a) return false;
+    std::lock_guard<std::mutex> lk(a->mu);
+    size_t off = (size_t)(static_cast<char*>(p) - a->base);
+    auto lv = a->live.find(off);
+    if (lv == a->live.end()) return true;   // not a live block: ignore
+    size_t len = lv->second;
+    a->live.erase(lv);
+    auto nx = a->free_at.lower_bound(off);
+    if (nx != a->free_at.end() && nx->first == off + len) {
+        len += nx->second;
+        arena_erase_free(a, nx);
+    }
+    auto pv = a->free_at.lower_bound(off);
+    if (pv != a->free_at.begin()) {
+        --pv;
+        if (pv->first + pv->second == off) {
+            off = pv->first;
+            len += pv->second;
+            arena_erase_free(a, pv);
+        }
+    }
+    arena_insert_free(a, off, len);
+    return true;
+}
+}  // namespace
+
+Arena* arena_create(size_t bytes) {
+    bytes &= ~size_t(255);
+    if (bytes == 0) return nullptr;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    Arena* a = new Arena();
+    a->base = static_cast<char*>(p);
+    a->size = bytes;
+    arena_insert_free(a, 0, bytes);
+    std::lock_guard<std::mutex> lk(g_arena_mu);
+    g_arenas.push_back(a);
+    return a;
+}
+
+void arena_destroy(Arena* a) {
+    if (!a) return;
+    {
+        std::lock_guard<std::mutex> lk(g_arena_mu);
+        g_arenas.erase(std::remove(g_arenas.begin(), g_arenas.end(), a), g_arenas.end());
+    }
+    (void)hipFree(a->base);
+    delete a;
+}
+
+void arena_bind(Arena* a) { t_arena = a; }
+
 hipError_t dev_alloc(void** out, size_t bytes) {
+    if (t_arena) {
+        // Arena first; a request it cannot fit falls back to the cache below.
+        void* p = arena_alloc(t_arena, std::max<size_t>(bytes, 256));
+        if (p) {
+            *out = p;
+            return hipSuccess;
+        }
+    }
     bytes = round_size(std::max<size_t>(bytes, 256));
     int dev = 0;
     (void)hipGetDevice(&dev);
@@ -111,6 +228,7 @@ hipError_t dev_alloc(void** out, size_t bytes) {
 
 void dev_free(void* p) {
     if (!p) return;
+    if (arena_free(p)) return;
     std::unique_lock<std::mutex> lk(g_alloc_mu);
     auto it = g_live.find(p);
     if (it == g_live.end()) {

@@ -15,6 +15,13 @@ namespace alls {
 hipError_t dev_alloc(void** out, size_t bytes);
 void dev_free(void* p);
 void dev_trim();
+// Device arena: one reservation carved up by the threads bound to it
+// (dev_alloc on such a thread takes from the arena first; dev_free of an
+// arena block from any thread returns it to the arena).
+struct Arena;
+Arena* arena_create(size_t bytes);   // nullptr when the reservation fails
+void arena_destroy(Arena* a);        // every block must have been freed
+void arena_bind(Arena* a);           // this thread's arena (nullptr: none)
 
 // Grow-only device buffer (owns its allocation; movable, not copyable).
 struct DevBuf {

@@ -5,9 +5,17 @@
 // stream and a workspace.  A request is admitted when a worker is idle and its
 // HBM footprint fits the remaining budget, so requests that arrive while
 // others run start (in arrival order) as soon as a slot and memory free up,
-// and their kernels overlap on the device.  Requests that carry a mesh key keep that mesh's
-// pattern on the device (LRU-evicted within the budget).  A repeat request
-// with a new E field runs only the value-only path.
+// and their kernels overlap on the device.  Requests that carry a mesh key can
+// keep that mesh's pattern on the device (LRU-evicted within the budget), so
+// a repeat request with a new E field runs only the value-only path.
+//
+// Building a pattern costs more than a plain assembly and a cached repeat
+// saves only part of one (A-LLS itself is close to the value-only path), so
+// by default a pattern is built only when the key's history pays for it:
+// with c earlier requests of the key taken as the expected number of future
+// repeats, build when c * (plain - hit) > build - plain, using per-element
+// costs of each path measured by the server itself (running averages per
+// dimension, seeded with typical MI355X figures).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -100,10 +108,31 @@ struct alls_server {
     std::vector<int64_t> ws_bytes;    // per worker, updated by that worker under the lock
     int64_t next_ticket = 1;
     uint64_t use_clock = 0;
+    std::unordered_map<uint64_t, int> seen;   // admitted requests per mesh key
+    // Per-element device cost (ns) of the plain, build and hit paths, per
+    // dimension (0: 2D, 1: Hex20), as running averages of served requests.
+    double cost[2][3] = {{0.7, 1.2, 0.55}, {35.0, 120.0, 30.0}};
+
+    enum { PATH_PLAIN = 0, PATH_BUILD = 1, PATH_HIT = 2 };
+    // Whether a request of a key seen `prior` times should build its pattern.
+    bool worth_building(const Request& q, int prior) const {
+        if (cfg.cache_policy == ALLS_CACHE_EAGER) return true;
+        const double* c = cost[q.mesh.dim == 3 ? 1 : 0];
+        return prior * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
+    }
+    // Folds one served request into the running per-element cost; small
+    // requests are dominated by fixed costs and are left out.
+    void observe(const Request& r, int path) {
+        if (r.mesh.n_elems < 50000 || r.status != ALLS_OK) return;
+        const double ns = ms_between(r.t_admit, r.t_done) * 1e6 / (double)r.mesh.n_elems;
+        double& c = cost[r.mesh.dim == 3 ? 1 : 0][path];
+        c = 0.8 * c + 0.2 * ns;
+    }
     bool stop = false;
     std::vector<std::thread> workers;
     std::vector<hipStream_t> streams;
     std::vector<std::unique_ptr<Workspace>> ws;
+    Arena* arena = nullptr;           // the budget, reserved once; workers allocate from it
 
     int64_t cache_bytes() const {
         int64_t b = 0;
@@ -125,6 +154,7 @@ struct alls_server {
 
     void worker(int slot) {
         (void)hipSetDevice(cfg.device);
+        arena_bind(arena);
         hipStream_t s = streams[slot];
         Workspace& w = *ws[slot];
         for (;;) {
@@ -147,8 +177,11 @@ struct alls_server {
                         const bool can_hit = c != cache.end() && c->second.precision == q->opts.precision &&
                                              c->second.n_elems == q->mesh.n_elems &&
                                              c->second.n_nodes == q->mesh.n_nodes;
+                        const auto sc = q->key ? seen.find(q->key) : seen.end();
+                        const int prior = sc == seen.end() ? 0 : sc->second;
                         const bool can_build = q->key && !can_hit && !building.count(q->key) &&
-                                               (c == cache.end() || c->second.refs == 0);
+                                               (c == cache.end() || c->second.refs == 0) &&
+                                               worth_building(*q, prior);
                         // This worker's workspace is reused; only its growth is new memory.
                         const int64_t grow = std::max<int64_t>(0, q->scratch - (int64_t)w.device_bytes());
                         const int64_t need = (can_build ? q->footprint : q->footprint - q->pattern_bytes) + grow;
@@ -165,6 +198,7 @@ struct alls_server {
                             }
                             r = q;
                             queue.pop_front();
+                            if (r->key) seen[r->key]++;
                             hit = can_hit;
                             build = can_build;
                             if (hit) {
@@ -245,6 +279,7 @@ struct alls_server {
                 for (int64_t b : ws_bytes) ws_total += b;
                 r->t_done = Clock::now();
                 r->status = rc;
+                observe(*r, hit ? PATH_HIT : build ? PATH_BUILD : PATH_PLAIN);
                 r->cache_hit = hit;
                 r->built = build && rc == ALLS_OK;
                 r->done = true;
@@ -281,6 +316,9 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
     // Default budget: 55% of free HBM; the allocator may keep up to a quarter
     // of HBM in freed blocks for reuse, so live + cached stays below capacity.
     s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)(0.55 * (double)free_b);
+    // Reserve the budget up front (workspaces, outputs and patterns are carved
+    // out of it); without the reservation the caching allocator serves them.
+    s->arena = arena_create((size_t)std::min<int64_t>(s->budget, (int64_t)free_b));
     for (int i = 0; i < n; ++i) {
         hipStream_t st;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
@@ -384,6 +422,7 @@ void alls_server_destroy(alls_server* s) {
     for (auto& kv : s->cache) pattern_destroy(kv.second.pat);
     for (auto& w : s->ws) w->destroy();
     for (auto st : s->streams) (void)hipStreamDestroy(st);
+    arena_destroy(s->arena);
     delete s;
 }
 

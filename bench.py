@@ -213,6 +213,93 @@ def hex20_compare(ctx, n, steps):
     return res
 
 
+def c3_compare(ctx, n, steps):
+    """Unstructured T3 and mixed Q4/T3 n x n (jitter 0.3h, random diagonals,
+    permuted node ids): A-LLS full path, value-only and GPU-coloured paths."""
+    import alls
+    res = {}
+    for name, kind, kw in (("t3", alls.GEN_T3, {}), ("mixed", alls.GEN_MIXED, {"split_fraction": 0.5})):
+        m = alls.generate(kind, n, n, jitter=0.3, seed=0xA11500C3, random_diagonal=1, permute_nodes=1, **kw)
+        dm = alls.DeviceMesh(m)
+        dt, st, out = time_steps(ctx, dm, steps, 1)
+        out.free()
+        r = {"elements": m.n_elems, "nnz": st["nnz"], "ms_per_step": round(dt / steps * 1e3, 3),
+             "elements_per_s": round(m.n_elems * steps / dt, 1), "ms_insert": round(st["ms_insert"], 3),
+             "ms_count": round(st["ms_count"], 3), "ms_write": round(st["ms_write"], 3), "big_rows": st["big_rows"]}
+        pat = alls.Pattern(ctx, m, c_mesh=dm.c_mesh())
+        pat.out.free()
+        dtv, o2 = time_calls(lambda o: pat.values(out=o), steps, 1)
+        r["value_only_ms"] = round(dtv * 1e3, 3)
+        o2.free()
+        col = pat.color_gpu()
+        dtc, o3 = time_calls(lambda o: pat.colored(col, out=o), steps, 1)
+        r["gpu_jpl_colors"] = col.n_colors
+        r["colored_ms"] = round(dtc * 1e3, 3)
+        r["alls_full_vs_colored_speedup"] = round(dtc / (dt / steps), 3)
+        o3.free()
+        col.free()
+        pat.free()
+        dm.free()
+        res[name] = r
+    res["workload"] = (f"{n}x{n} grid, jitter +-0.3h, random diagonal per cell, node ids randomly permuted; "
+                       "T3: every cell split; mixed: 50% of cells split; E=1, nu=0.3, FP64")
+    return res
+
+
+def batch_compare(ctx, n_meshes=256, seed=7):
+    """Many small meshes: one alls_assemble_batch call (packed pipeline,
+    persistent insertion, replayed hipGraphs) vs one alls_assemble per mesh."""
+    import alls
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for i in range(n_meshes):
+        kind = (alls.GEN_Q4, alls.GEN_T3, alls.GEN_MIXED)[i % 3]
+        size = float(np.exp(rng.uniform(np.log(1e3), np.log(3e4))))
+        nx = max(2, int(np.sqrt(size / (2.0 if kind == alls.GEN_T3 else 1.0))))
+        meshes.append(alls.generate(kind, nx, nx, jitter=0.2, seed=seed + i, random_diagonal=1, E_sigma=0.5))
+    n_el = sum(m.n_elems for m in meshes)
+    dms = [alls.DeviceMesh(m) for m in meshes]
+    cms = [d.c_mesh() for d in dms]
+    out = None
+    for _ in range(2):   # first call captures the graphs
+        out, _, _ = ctx.assemble_batch(meshes, out=out, c_meshes=cms)
+    alls.hip().sync()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out, _, _ = ctx.assemble_batch(meshes, out=out, c_meshes=cms)
+    t_batch = (time.perf_counter() - t0) / reps
+    out.free()
+    outs = [None] * n_meshes
+    for k in range(n_meshes):
+        outs[k] = ctx.assemble(meshes[k], out=outs[k], c_mesh=cms[k])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for k in range(n_meshes):
+            outs[k] = ctx.assemble(meshes[k], out=outs[k], c_mesh=cms[k])
+    t_seq = (time.perf_counter() - t0) / reps
+    for o in outs:
+        o.free()
+    for d in dms:
+        d.free()
+    return {"meshes": n_meshes, "elements": n_el, "batch_ms": round(t_batch * 1e3, 3),
+            "one_by_one_ms": round(t_seq * 1e3, 3), "batch_elements_per_s": round(n_el / t_batch, 1),
+            "speedup": round(t_seq / t_batch, 2),
+            "workload": f"{n_meshes} 2D meshes (Q4, T3, mixed in turn), 1e3-3e4 elements log-uniform, device-resident"}
+
+
+def autotune_report(ctx, dmesh, steps, default_ms):
+    """alls_autotune on the main workload and the tuned setting's time."""
+    import alls
+    t0 = time.perf_counter()
+    best = ctx.autotune(dmesh.mesh, c_mesh=dmesh.c_mesh())
+    t_tune = time.perf_counter() - t0
+    dt, st, out = time_steps(ctx, dmesh, steps, 1, **best)
+    out.free()
+    return {"setting": best, "tuning_s": round(t_tune, 2), "tuned_ms_per_step": round(dt / steps * 1e3, 4),
+            "default_ms_per_step": round(default_ms, 4)}
+
+
 def serve_costs_from(res, extra, n_q4):
     """Per-element service times (ms) measured earlier in this run."""
     import serve_mix
@@ -259,6 +346,7 @@ def main():
     ap.add_argument("--n", type=int, default=2000, help="Q4 grid size (workload n x n)")
     ap.add_argument("--cpu-n", type=int, default=1000, help="CPU baseline sample grid size")
     ap.add_argument("--hex-n", type=int, default=80)
+    ap.add_argument("--c3-n", type=int, default=1500, help="T3 / mixed grid size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
@@ -309,7 +397,24 @@ def main():
             extra["coloring_q4"] = coloring_compare(ctx, mesh, dm, a.steps, ms)
         except Exception as e:
             extra["coloring_q4"] = {"error": str(e)}
+    if not a.no_extra:
+        try:
+            log("autotune on the main workload")
+            extra["autotune_q4"] = autotune_report(ctx, dm, a.steps, ms)
+        except Exception as e:
+            extra["autotune_q4"] = {"error": str(e)}
     dm.free()
+    if not a.no_extra:
+        try:
+            log("batched small meshes")
+            extra["batch"] = batch_compare(ctx)
+        except Exception as e:
+            extra["batch"] = {"error": str(e)}
+        try:
+            log(f"T3 / mixed {a.c3_n}x{a.c3_n}")
+            extra["c3"] = c3_compare(ctx, a.c3_n, 5)
+        except Exception as e:
+            extra["c3"] = {"error": str(e)}
     if not a.no_cpu:
         log(f"cpu baseline on Q4 {a.cpu_n}x{a.cpu_n}")
         res["cpu_baseline"] = cpu_baseline(a.cpu_n)

@@ -225,13 +225,20 @@ alls_status alls_autotune(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
  * in arrival order as soon as a stream is idle and their HBM footprint fits
  * the budget, run concurrently on up to n_streams streams, and reuse cached
  * patterns when mesh_key repeats. */
+/* When a request with a mesh key builds and caches its pattern.  COST (the
+ * default): when the key's earlier requests, taken as the expected number of
+ * repeats, times the saving of a cached repeat exceed the extra cost of the
+ * build, with per-element costs the server measures (so never on a key's
+ * first request).  EAGER: on the first request of every key. */
+typedef enum { ALLS_CACHE_COST = 0, ALLS_CACHE_EAGER = 1 } alls_cache_policy;
+
 typedef struct {
     int32_t device;
     int32_t n_streams;        /* concurrent in-flight requests; 0 => 2 (more streams share the
                                  device's 4 hardware queues and lose to allocation stalls) */
     int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM */
     int32_t pattern_cache_entries;
-    int32_t reserved;
+    int32_t cache_policy;     /* alls_cache_policy */
 } alls_server_config;
 
 typedef struct {
@@ -251,7 +258,8 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** srv)
 alls_status alls_server_submit(alls_server* srv, const alls_mesh* mesh, uint64_t mesh_key,
                                const alls_opts* opts, int64_t* ticket);
 /* Blocks until the request completes.  Its CSR moves into *out (the caller
- * frees it with alls_server_release). */
+ * frees it with alls_server_release, before alls_server_destroy: outputs live
+ * in the server's HBM reservation). */
 alls_status alls_server_wait(alls_server* srv, int64_t ticket, alls_csr* out, alls_request_stats* st);
 void        alls_server_release(alls_server* srv, alls_csr* csr);
 void        alls_server_destroy(alls_server* srv);

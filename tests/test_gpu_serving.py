@@ -29,7 +29,7 @@ def mix_requests():
 
 def test_mix_equals_alone_and_oracle(ctx):
     reqs = mix_requests()
-    with alls.Server(n_streams=4) as srv:
+    with alls.Server(n_streams=4, cache_policy=alls.CACHE_EAGER) as srv:
         tickets = [srv.submit(m, key=key) for key, m in reqs]
         results = []
         for t in tickets:
@@ -51,7 +51,7 @@ def test_mix_equals_alone_and_oracle(ctx):
 def test_value_only_hits_are_bitwise_repeatable():
     m = alls.generate(alls.GEN_Q4, 50, 50, jitter=0.2, seed=9)
     E = np.exp(0.5 * np.random.default_rng(2).standard_normal(m.n_elems))
-    with alls.Server(n_streams=2) as srv:
+    with alls.Server(n_streams=2, cache_policy=alls.CACHE_EAGER) as srv:
         srv.wait(srv.submit(m, key=7), keep_output=False)
         outs = []
         for _ in range(3):
@@ -60,6 +60,26 @@ def test_value_only_hits_are_bitwise_repeatable():
             outs.append(srv.download(csr)[2])
             srv.release(csr)
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_cost_policy_builds_only_when_repeats_pay_for_it():
+    # Default policy with the seeded 2D costs (plain 0.7, build 1.2, hit 0.55
+    # ns/element; small requests do not update them): a key builds its
+    # pattern once 4 earlier requests predict enough repeats (4 * 0.15 > 0.5),
+    # and hits after that.  Every result equals the plain assembly.
+    m = alls.generate(alls.GEN_Q4, 30, 20, jitter=0.2, seed=5)
+    rng = np.random.default_rng(3)
+    kinds, outs = [], []
+    with alls.Server(n_streams=2) as srv:
+        for _ in range(7):
+            mk = m.with_E(np.exp(0.5 * rng.standard_normal(m.n_elems)))
+            csr, st = srv.wait(srv.submit(mk, key=11))
+            kinds.append(st["cache_hit"])
+            outs.append((mk, srv.download(csr)))
+            srv.release(csr)
+    assert kinds == [0, 0, 0, 0, 2, 1, 1]
+    for mk, got in outs:
+        compare_csr(got, oracle.assemble(mk))
 
 
 def test_small_budget_serializes_and_refuses_oversized():
