@@ -72,6 +72,7 @@ class RequestStats(C.Structure):
     _fields_ = [
         ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
         ("nnz", C.c_int64), ("cache_hit", C.c_int32), ("status", C.c_int32),
+        ("batch_size", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

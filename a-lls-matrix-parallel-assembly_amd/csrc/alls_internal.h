@@ -220,6 +220,16 @@ struct PackMesh {
 // Packs nm device meshes into coords/conn (and types, E when non-null).
 void launch_pack(const PackMesh* pm, int nm, int dim, int npe, int64_t n_nodes, int64_t n_elems, double* coords,
                  int32_t* conn, uint8_t* types, double* E, hipStream_t s);
+// One row/nonzero range of a block-diagonal batch result, copied into its
+// own CSR with local row and column numbers.
+struct SplitPart {
+    int64_t row_off, nnz_off, n_rows, nnz;
+    int32_t* row_ptr;           // int32 [n_rows+1]
+    int32_t* col_ind;
+    double* val;
+};
+void launch_split(const SplitPart* parts, int n, int64_t total_rows, int64_t total_nnz, const void* row_ptr,
+                  int i64, const int32_t* col_ind, const double* val, hipStream_t s);
 // out[i] = row_ptr[idx[i]] as int64.
 void launch_gather_rp(const void* row_ptr, int i64, const int64_t* idx, int n, int64_t* out, hipStream_t s);
 

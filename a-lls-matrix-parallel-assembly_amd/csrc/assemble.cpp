@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -279,6 +280,28 @@ hipError_t Workspace::init() {
     return hipSuccess;
 }
 
+// Host copy into the pinned staging buffer.  Large inputs are split over a
+// few threads: one thread's memcpy (about 5-10 GB/s) would otherwise cost
+// more than the GPU work of the assembly it feeds.
+static void staged_copy(void* dst, const void* src, size_t n) {
+    constexpr size_t kSplit = size_t(32) << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t parts = std::min<size_t>(std::min<size_t>(8, hw), n / kSplit);
+    if (parts < 2) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t step = ((n + parts - 1) / parts + 4095) & ~size_t(4095);
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < parts && k * step < n; ++k)
+        th.emplace_back([=] {
+            std::memcpy(static_cast<char*>(dst) + k * step, static_cast<const char*>(src) + k * step,
+                        std::min(step, n - k * step));
+        });
+    std::memcpy(dst, src, std::min(step, n));
+    for (auto& t : th) t.join();
+}
+
 hipError_t Workspace::staging(size_t total, char** host) {
     *host = nullptr;
     // The previous upload from this buffer has completed: every pipeline
@@ -323,7 +346,7 @@ hipError_t Workspace::upload(void* const* dst, const void* const* src, const siz
     for (int i = 0; i < n; ++i) {
         if (!bytes[i]) continue;
         char* stage = static_cast<char*>(pinned) + off;
-        std::memcpy(stage, src[i], bytes[i]);
+        staged_copy(stage, src[i], bytes[i]);
         hipError_t e = hipMemcpyAsync(dst[i], stage, bytes[i], hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
         off += (bytes[i] + 255) & ~size_t(255);
@@ -333,7 +356,7 @@ hipError_t Workspace::upload(void* const* dst, const void* const* src, const siz
 
 size_t Workspace::device_bytes() const {
     size_t b = 0;
-    for (const DevBuf* x : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc})
+    for (const DevBuf* x : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b += x->bytes;
     return b;
 }
@@ -341,14 +364,14 @@ size_t Workspace::device_bytes() const {
 void Workspace::release_device() {
     g_front.reset();
     g_back.reset();
-    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc})
+    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b->release();
 }
 
 void Workspace::destroy() {
     g_front.reset();
     g_back.reset();
-    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc})
+    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b->release();
     if (host_misc) (void)hipHostFree(host_misc);
     host_misc = nullptr;

@@ -132,13 +132,24 @@ alls_status alls_autotune(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
 alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshes, const alls_opts* opts,
                                 alls_csr* out, int64_t* row_off, int64_t* nnz_off) {
     if (!ctx || n < 1 || !meshes || !out || !row_off || !nnz_off) return ALLS_E_INVALID;
-    std::string* err = &ctx->err;
-    err->clear();
+    ctx->err.clear();
     (void)hipSetDevice(ctx->device);
     alls_opts o{};
     if (opts) o = *opts;
-    Workspace& ws = ctx->ws;
-    hipStream_t s = ctx->stream;
+    alls_stats stats{};
+    const alls_status st = batch_assemble(ctx->ws, ctx->stream, n, meshes, o, out, row_off, nnz_off, &stats,
+                                          &ctx->err, ctx, /*use_graph=*/true);
+    if (st == ALLS_OK) ctx->stats = stats;
+    return st;
+}
+
+}  // extern "C"
+
+namespace alls {
+
+alls_status batch_assemble(Workspace& ws, hipStream_t s, int32_t n, const alls_mesh* meshes, alls_opts o,
+                           alls_csr* out, int64_t* row_off, int64_t* nnz_off, alls_stats* stats_out,
+                           std::string* err, alls_ctx* tune_ctx, bool use_graph) {
 
     // Validate and lay out the packed mesh.
     std::vector<MeshInfo> mis((size_t)n);
@@ -265,10 +276,11 @@ alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshe
     dm.npe = npe;
     dm.uniform_type = uniform;
     if (o.grid_per_cu <= 0 && !o.autotune) o.grid_per_cu = dim == 3 ? 4 : 8;   // persistent by default
-    alls_status st = apply_tuning(ctx, dm, mi, &o);
+    alls_status st = ALLS_OK;
+    if (tune_ctx) st = apply_tuning(tune_ctx, dm, mi, &o);
     if (st != ALLS_OK) return st;
     alls_stats stats{};
-    st = run_alls(ws, dm, mi, o, out, s, &stats, err, /*use_graph=*/N > 0);
+    st = run_alls(ws, dm, mi, o, out, s, &stats, err, use_graph && N > 0);
     if (st != ALLS_OK) return st;
 
     // Offsets of each mesh's rows and nonzeros in the block-diagonal result.
@@ -288,8 +300,8 @@ alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshe
     HIP_TRY(hipStreamSynchronize(s), "sync");
     std::memcpy(nnz_off, hres, (size_t)(n + 1) * sizeof(int64_t));
     stats.n_elems = Ne;
-    ctx->stats = stats;
+    if (stats_out) *stats_out = stats;
     return ALLS_OK;
 }
 
-}  // extern "C"
+}  // namespace alls

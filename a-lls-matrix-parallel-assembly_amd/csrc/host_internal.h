@@ -69,6 +69,7 @@ struct Workspace {
     DevBuf hdr, keys, vals, misc, u, big, tiles;
     DevBuf coords, conn, types, E;
     DevBuf bdesc;                   // batch: source-mesh table, boundary indices and offsets
+    DevBuf bsplit;                  // batch split: per-request output table (server)
     int64_t* host_misc = nullptr;   // pinned readback: misc ints + scan total
     void* pinned = nullptr;         // pinned staging of host inputs (grow-only)
     size_t pinned_bytes = 0;
@@ -131,4 +132,10 @@ namespace alls {
 // opts.autotune: copies the tuned block_threads / chunk_slots / grid_per_cu
 // of the mesh's kind and size class into *o, tuning on dm first when new.
 alls_status apply_tuning(alls_ctx* ctx, const DevMesh& dm, const MeshInfo& mi, alls_opts* o);
+// n meshes of one dimension in one packed pipeline on ws/s (alls_assemble_batch);
+// tune_ctx (may be null) serves opts.autotune; use_graph replays captured
+// graphs (worth it when the same batch shape repeats).
+alls_status batch_assemble(Workspace& ws, hipStream_t s, int32_t n, const alls_mesh* meshes, alls_opts o,
+                           alls_csr* out, int64_t* row_off, int64_t* nnz_off, alls_stats* stats, std::string* err,
+                           alls_ctx* tune_ctx, bool use_graph);
 }  // namespace alls

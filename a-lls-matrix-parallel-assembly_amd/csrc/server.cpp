@@ -53,6 +53,7 @@ struct Request {
     bool done = false;
     bool cache_hit = false;
     bool built = false;
+    int batch_size = 1;
     alls_csr out{};
     std::string err;
 };
@@ -93,6 +94,45 @@ int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pat
     return out + pat;
 }
 
+// Runs reqs (plain, one dimension) as one packed batch on w/s and gives each
+// request its own CSR: the block-diagonal result is split by one kernel.
+alls_status run_batch(Workspace& w, hipStream_t s, const std::vector<std::shared_ptr<Request>>& reqs,
+                      alls_stats* st, std::string* err) {
+    const int n = (int)reqs.size();
+    std::vector<alls_mesh> ms((size_t)n);
+    for (int k = 0; k < n; ++k) ms[k] = reqs[k]->mesh;
+    std::vector<int64_t> ro((size_t)n + 1), no((size_t)n + 1);
+    alls_csr bout{};
+    // Every batch has its own shape, so no graph capture (it would not replay).
+    alls_status rc = batch_assemble(w, s, n, ms.data(), reqs[0]->opts, &bout, ro.data(), no.data(), st, err, nullptr,
+                                    false);
+    std::vector<SplitPart> parts((size_t)n);
+    for (int k = 0; k < n && rc == ALLS_OK; ++k) {
+        rc = ensure_csr(&reqs[k]->out, ro[k + 1] - ro[k], no[k + 1] - no[k], 0);
+        parts[k] = SplitPart{ro[k], no[k], ro[k + 1] - ro[k], no[k + 1] - no[k],
+                             static_cast<int32_t*>(reqs[k]->out.row_ptr), reqs[k]->out.col_ind, reqs[k]->out.val};
+    }
+    if (rc == ALLS_OK) {
+        const size_t tb = (size_t)n * sizeof(SplitPart);
+        char* h = nullptr;
+        hipError_t e = w.staging(tb, &h);
+        if (e == hipSuccess) e = w.bsplit.ensure(tb);
+        if (e == hipSuccess) {
+            std::memcpy(h, parts.data(), tb);
+            e = hipMemcpyAsync(w.bsplit.p, h, tb, hipMemcpyHostToDevice, s);
+        }
+        if (e == hipSuccess) {
+            launch_split(w.bsplit.as<SplitPart>(), n, ro[n], no[n], bout.row_ptr, bout.row_ptr_is_i64, bout.col_ind,
+                         bout.val, s);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = hip_fail(e, "batch split", err);
+    }
+    free_csr(&bout);
+    return rc;
+}
+
 }  // namespace
 
 struct alls_server {
@@ -120,6 +160,29 @@ struct alls_server {
         const double* c = cost[q.mesh.dim == 3 ? 1 : 0];
         return prior * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
     }
+    // Small plain requests queued right behind a plain head join it in one
+    // packed batch (one pipeline, persistent insertion, split afterwards).
+    static constexpr int kBatchMax = 256;
+    static int64_t batch_elems_max(int dim) { return dim == 3 ? 65536 : 1 << 20; }
+    static bool batchable(const Request& q) {
+        return q.mesh.n_elems > 0 && q.mesh.n_elems <= (q.mesh.dim == 3 ? 16384 : 131072);
+    }
+    static bool compatible(const Request& a, const Request& b) {
+        return a.mesh.dim == b.mesh.dim && a.mesh.mem == b.mesh.mem && a.mesh.nu == b.mesh.nu &&
+               (a.mesh.dim == 3 || a.mesh.thickness == b.mesh.thickness) &&
+               (a.mesh.dim == 2 || a.opts.precision == b.opts.precision);
+    }
+    // Whether queued request q would run the plain path if it were the head.
+    bool runs_plain(const Request& q) const {
+        if (!q.key) return true;
+        auto c = cache.find(q.key);
+        if (c != cache.end() && c->second.precision == q.opts.precision && c->second.n_elems == q.mesh.n_elems &&
+            c->second.n_nodes == q.mesh.n_nodes)
+            return false;
+        auto sc = seen.find(q.key);
+        return building.count(q.key) || !worth_building(q, sc == seen.end() ? 0 : sc->second);
+    }
+
     // Folds one served request into the running per-element cost; small
     // requests are dominated by fixed costs and are left out.
     void observe(const Request& r, int path) {
@@ -159,6 +222,7 @@ struct alls_server {
         Workspace& w = *ws[slot];
         for (;;) {
             std::shared_ptr<Request> r;
+            std::vector<std::shared_ptr<Request>> batch;   // plain requests batched behind r
             bool hit = false, build = false;
             CacheEntry* ce = nullptr;
             {
@@ -221,8 +285,29 @@ struct alls_server {
                     if (r) break;
                     cv_work.wait(lk);
                 }
+                if (!hit && !build && batchable(*r)) {
+                    int64_t elems = r->mesh.n_elems;
+                    while (!queue.empty() && (int)batch.size() + 1 < kBatchMax) {
+                        std::shared_ptr<Request> q = queue.front();
+                        if (!batchable(*q) || !compatible(*r, *q) || !runs_plain(*q) ||
+                            elems + q->mesh.n_elems > batch_elems_max(r->mesh.dim))
+                            break;
+                        // Its output, the block-diagonal copy and its share of the
+                        // batch's scratch are charged while it runs.
+                        const int64_t out_b = q->footprint - q->pattern_bytes;
+                        const int64_t need = 2 * out_b + q->scratch;
+                        if (in_use + ws_total + cache_bytes() + need > budget) break;
+                        queue.pop_front();
+                        if (q->key) seen[q->key]++;
+                        q->footprint = need;
+                        in_use += need;
+                        elems += q->mesh.n_elems;
+                        batch.push_back(q);
+                    }
+                }
             }
             r->t_admit = Clock::now();
+            for (auto& q : batch) q->t_admit = r->t_admit;
             alls_stats st{};
             alls_status rc;
             if (hit) {
@@ -265,6 +350,10 @@ struct alls_server {
                     }
                     cache[r->key] = e;
                 }
+            } else if (!batch.empty()) {
+                batch.insert(batch.begin(), r);
+                rc = run_batch(w, s, batch, &st, &r->err);
+                batch.erase(batch.begin());
             } else {
                 MeshInfo mi;
                 DevMesh dm;
@@ -279,7 +368,7 @@ struct alls_server {
                 for (int64_t b : ws_bytes) ws_total += b;
                 r->t_done = Clock::now();
                 r->status = rc;
-                observe(*r, hit ? PATH_HIT : build ? PATH_BUILD : PATH_PLAIN);
+                if (batch.empty()) observe(*r, hit ? PATH_HIT : build ? PATH_BUILD : PATH_PLAIN);
                 r->cache_hit = hit;
                 r->built = build && rc == ALLS_OK;
                 r->done = true;
@@ -289,6 +378,17 @@ struct alls_server {
                 in_use -= r->footprint;
                 r->footprint = rc == ALLS_OK ? outb : 0;
                 in_use += r->footprint;
+                if (!batch.empty()) r->batch_size = (int)batch.size() + 1;
+                for (auto& q : batch) {
+                    q->batch_size = r->batch_size;
+                    q->t_done = r->t_done;
+                    q->status = rc;
+                    if (rc != ALLS_OK) q->err = r->err;
+                    q->done = true;
+                    in_use -= q->footprint;
+                    q->footprint = rc == ALLS_OK ? q->out.cap_nnz * 12 + q->out.cap_rows * 8 : 0;
+                    in_use += q->footprint;
+                }
             }
             cv_done.notify_all();
             cv_work.notify_all();
@@ -382,6 +482,8 @@ alls_status alls_server_wait(alls_server* s, int64_t ticket, alls_csr* out, alls
         st->nnz = r->out.nnz;
         st->cache_hit = r->cache_hit ? 1 : r->built ? 2 : 0;
         st->status = r->status;
+        st->batch_size = r->batch_size;
+        st->reserved = 0;
     }
     if (out) {
         *out = r->out;

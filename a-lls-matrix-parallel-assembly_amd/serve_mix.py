@@ -112,6 +112,8 @@ def summarize(reqs, stats, makespan, errors):
     lat = np.array([s["latency_ms"] for s in stats if s is not None])
     q = np.array([s["queue_ms"] for s in stats if s is not None])
     hits = sum(1 for s in stats if s is not None and s["cache_hit"] == 1)
+    bsz = [s.get("batch_size", 1) for s in stats if s is not None]
+    batched = sum(1 for b in bsz if b > 1)
     paths = {}
     for (_, m), st in zip(reqs, stats):
         if st is None:
@@ -128,5 +130,7 @@ def summarize(reqs, stats, makespan, errors):
     return {"requests": len(reqs), "completed": int(lat.size), "refused": len(errors),
             "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "mean_ms": round(float(lat.mean()), 3), "queue_p99_ms": round(float(np.percentile(q, 99)), 3),
-            "cache_hits": hits, "paths": paths, "elements": int(elems), "makespan_s": round(makespan, 3),
+            "cache_hits": hits, "batched_requests": batched,
+            "mean_batch_size_of_batched": round(float(np.mean([b for b in bsz if b > 1])), 2) if batched else 0.0,
+            "paths": paths, "elements": int(elems), "makespan_s": round(makespan, 3),
             "elements_per_s": round(elems / makespan, 1)}

@@ -156,6 +156,9 @@ def coloring_compare(ctx, mesh, dm, steps, alls_ms):
     st = ctx.stats()
     res["value_only_device_ms"] = round(st["ms_total"], 4)
     out.free()
+    dt, out = time_calls(lambda o: pat.values(E_per_elem=mesh.E_per_elem, out=o), steps, 1)
+    res["value_only_host_E_ms"] = round(dt * 1e3, 4)
+    out.free()
     for name, mk in (("structured_4color", lambda: pat.color_from_host(mesh.structured_color)),
                      ("gpu_jpl", lambda: pat.color_gpu())):
         t0 = time.perf_counter()
@@ -188,6 +191,8 @@ def hex20_compare(ctx, n, steps):
                      "ms_insert": round(st["ms_insert"], 3), "ms_count": round(st["ms_count"], 3),
                      "ms_write": round(st["ms_write"], 3), "nnz": st["nnz"]}
         v[name] = out.download()[2]
+        dt_h, out = time_calls(lambda o: ctx.assemble(m, precision=prec, out=o), 2, 1)
+        res[name]["host_input_ms"] = round(dt_h * 1e3, 3)
         out.free()
     # Cached pattern: value-only re-assembly (staged, no atomics) and the
     # coloured path with the 8-colour cell-parity colouring.
@@ -196,6 +201,9 @@ def hex20_compare(ctx, n, steps):
         pat.out.free()
         dt, out = time_calls(lambda o: pat.values(out=o), steps, 1)
         res[name]["value_only_ms"] = round(dt * 1e3, 3)
+        Eh = np.full(m.n_elems, 1.0)
+        dt, out = time_calls(lambda o: pat.values(E_per_elem=Eh, out=o), 2, 1)
+        res[name]["value_only_host_E_ms"] = round(dt * 1e3, 3)
         if name == "f64":
             col = pat.color_from_host(m.structured_color)
             dtc, out2 = time_calls(lambda o: pat.colored(col, out=o), steps, 1)
@@ -301,18 +309,20 @@ def autotune_report(ctx, dmesh, steps, default_ms):
 
 
 def serve_costs_from(res, extra, n_q4):
-    """Per-element service times (ms) measured earlier in this run."""
+    """Per-element service times (ms) measured earlier in this run, with the
+    inputs in host memory as the served requests have them."""
     import serve_mix
     c = dict(serve_mix.DEFAULT_COSTS)
-    c["2d_full"] = res["ms_per_step"] / n_q4
+    c["2d_full"] = res.get("host_input", {}).get("ms_per_step", res["ms_per_step"]) / n_q4
     cq = extra.get("coloring_q4", {})
     if "value_only_ms" in cq:
-        c["2d_hit"] = cq["value_only_ms"] / n_q4
+        c["2d_hit"] = cq.get("value_only_host_E_ms", cq["value_only_ms"]) / n_q4
     hx = extra.get("hex20", {})
     if "f64" in hx:
-        n_hex = hx["f64"]["elements_per_s"] * hx["f64"]["ms_per_step"] / 1e3
-        c["hex20_full"] = hx["f64"]["ms_per_step"] / n_hex
-        c["hex20_hit"] = hx["f64"].get("value_only_ms", hx["f64"]["ms_per_step"]) / n_hex
+        f = hx["f64"]
+        n_hex = f["elements_per_s"] * f["ms_per_step"] / 1e3
+        c["hex20_full"] = f.get("host_input_ms", f["ms_per_step"]) / n_hex
+        c["hex20_hit"] = f.get("value_only_host_E_ms", f.get("value_only_ms", f["ms_per_step"])) / n_hex
     return c
 
 
@@ -373,6 +383,8 @@ def main():
     pb = path_bytes(mesh, st["nnz"])
     log(f"{value:.3e} elements/s, {ms:.3f} ms/step, stages {times}")
     out.free()
+    dt_h, out = time_calls(lambda o: ctx.assemble(mesh, out=o), max(3, a.steps // 2), 1)
+    out.free()
     res = {
         "metric": METRIC, "value": round(value, 1), "unit": "elements/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
@@ -389,6 +401,9 @@ def main():
         "stages_ms": {k: round(v, 4) for k, v in times.items()},
         "path": {"compulsory_bytes": pb, "achieved_GBs": round(pb / (ms * 1e-3) / 1e9, 1),
                  "frac_of_hbm_peak": round(pb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "host_input": {"ms_per_step": round(dt_h * 1e3, 4), "elements_per_s": round(mesh.n_elems / dt_h, 1),
+                       "note": "same workload with the mesh in host memory: pinned staging and the PCIe "
+                               "copy inside the timed region (not the metric's value)"},
     }
     extra = {}
     if not a.no_extra:

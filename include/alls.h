@@ -249,6 +249,8 @@ typedef struct {
     int32_t cache_hit;        /* 1 => value-only assembly on a cached pattern, 2 => this request
                                  built and cached the pattern, 0 => plain assembly */
     int32_t status;           /* alls_status of the request */
+    int32_t batch_size;       /* requests assembled in the same packed launch (1: alone) */
+    int32_t reserved;
 } alls_request_stats;
 
 alls_status alls_server_create(const alls_server_config* cfg, alls_server** srv);

@@ -82,6 +82,33 @@ def test_cost_policy_builds_only_when_repeats_pay_for_it():
         compare_csr(got, oracle.assemble(mk))
 
 
+def test_queued_small_requests_are_batched_and_exact(ctx):
+    # Submitted at once, the small plain requests queue up and run in packed
+    # batches behind the head; each result must equal its plain assembly.
+    rng = np.random.default_rng(8)
+    reqs = []
+    for i in range(60):
+        kind = (alls.GEN_Q4, alls.GEN_T3, alls.GEN_MIXED)[i % 3]
+        n = int(rng.integers(3, 30))
+        reqs.append(alls.generate(kind, n, n + 2, jitter=0.2, seed=100 + i, random_diagonal=1, E_sigma=0.5))
+    hex_reqs = [alls.generate(alls.GEN_HEX20, 2, 2, 1 + i % 3, jitter=0.1, seed=200 + i) for i in range(6)]
+    with alls.Server(n_streams=2) as srv:
+        tickets = [srv.submit(m) for m in reqs + hex_reqs]
+        got = []
+        for t in tickets:
+            csr, st = srv.wait(t)
+            got.append((srv.download(csr), st))
+            srv.release(csr)
+    assert max(st["batch_size"] for _, st in got) > 1
+    for m, (g, st) in zip(reqs + hex_reqs, got):
+        assert st["status"] == 0
+        alone = ctx.assemble(m).download()
+        for a, b in zip(g, alone):
+            assert np.array_equal(a, b)
+    for m, (g, _) in list(zip(reqs, got))[:6]:
+        compare_csr(g, oracle.assemble(m))
+
+
 def test_small_budget_serializes_and_refuses_oversized():
     small = alls.generate(alls.GEN_Q4, 40, 40, seed=1)
     big = alls.generate(alls.GEN_HEX20, 12, 12, 12, seed=1)
