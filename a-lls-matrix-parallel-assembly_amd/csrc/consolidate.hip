@@ -129,11 +129,22 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     if (tail) {
         const uint64_t le = wl == 63 ? ~0ull : ((1ull << (wl + 1)) - 1);
         const int32_t rank = __popcll(gheads & le) - 1;
-        for (int dr = 0; dr < dpn; ++dr) {
-            const int64_t rs = (int64_t)row_ptr[a * dpn + dr] + (int64_t)rank * dpn;
-            for (int dc = 0; dc < dpn; ++dc) {
-                col_ind[rs + dc] = col * dpn + dc;
-                val[rs + dc] = v[dr * dpn + dc];
+        if constexpr (B == 4) {
+            // 2 DOF per node: row_ptr values are even, so each block row is one
+            // aligned int2 + double2 store.
+#pragma unroll
+            for (int dr = 0; dr < 2; ++dr) {
+                const int64_t rs = (int64_t)row_ptr[a * 2 + dr] + (int64_t)rank * 2;
+                *reinterpret_cast<int2*>(col_ind + rs) = make_int2(col * 2, col * 2 + 1);
+                *reinterpret_cast<double2*>(val + rs) = make_double2(v[dr * 2], v[dr * 2 + 1]);
+            }
+        } else {
+            for (int dr = 0; dr < dpn; ++dr) {
+                const int64_t rs = (int64_t)row_ptr[a * dpn + dr] + (int64_t)rank * dpn;
+                for (int dc = 0; dc < dpn; ++dc) {
+                    col_ind[rs + dc] = col * dpn + dc;
+                    val[rs + dc] = v[dr * dpn + dc];
+                }
             }
         }
     }

@@ -359,11 +359,14 @@ def main():
     ap.add_argument("--c3-n", type=int, default=1500, help="T3 / mixed grid size")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--extras", default="coloring,autotune,batch,c3,hex20,serve",
+                    help="comma list of secondary measurements to run (or 'none')")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="end with one device-to-device copy of BYTES (a copy kernel of known size, the "
                          "scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     a = ap.parse_args()
+    ex = set() if a.no_extra or a.extras == "none" else set(a.extras.split(","))
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the multi-GPU path is not built.",
               file=sys.stderr)
@@ -406,25 +409,26 @@ def main():
                                "copy inside the timed region (not the metric's value)"},
     }
     extra = {}
-    if not a.no_extra:
+    if "coloring" in ex:
         log("coloring / value-only comparison")
         try:
             extra["coloring_q4"] = coloring_compare(ctx, mesh, dm, a.steps, ms)
         except Exception as e:
             extra["coloring_q4"] = {"error": str(e)}
-    if not a.no_extra:
+    if "autotune" in ex:
         try:
             log("autotune on the main workload")
             extra["autotune_q4"] = autotune_report(ctx, dm, a.steps, ms)
         except Exception as e:
             extra["autotune_q4"] = {"error": str(e)}
     dm.free()
-    if not a.no_extra:
+    if "batch" in ex:
         try:
             log("batched small meshes")
             extra["batch"] = batch_compare(ctx)
         except Exception as e:
             extra["batch"] = {"error": str(e)}
+    if "c3" in ex:
         try:
             log(f"T3 / mixed {a.c3_n}x{a.c3_n}")
             extra["c3"] = c3_compare(ctx, a.c3_n, 5)
@@ -434,14 +438,14 @@ def main():
         log(f"cpu baseline on Q4 {a.cpu_n}x{a.cpu_n}")
         res["cpu_baseline"] = cpu_baseline(a.cpu_n)
         res["cpu_baseline"]["gpu_speedup"] = round(value / res["cpu_baseline"]["value"], 1)
-    if not a.no_extra:
+    if "hex20" in ex:
         try:
             log(f"Hex20 {a.hex_n}^3 FP64 vs FP32-MFMA")
             extra["hex20"] = hex20_compare(ctx, a.hex_n, 3)
         except Exception as e:  # report, do not hide the main line
             extra["hex20"] = {"error": str(e)}
     ctx.close()   # frees the context's scratch before the server takes its HBM budget
-    if not a.no_extra and a.serve_requests > 0:
+    if "serve" in ex and a.serve_requests > 0:
         try:
             log(f"served mix: {a.serve_requests} requests")
             costs = dict(serve_costs_from(res, extra, mesh.n_elems), fixed=0.2)
