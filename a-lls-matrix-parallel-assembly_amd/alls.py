@@ -116,6 +116,9 @@ SIGNATURES = {
     "alls_assemble_colored": (C.c_int, [vp, vp, vp, vp, C.c_int32, P(Csr)]),
     "alls_assemble_batch": (C.c_int, [vp, C.c_int32, P(Mesh), P(Opts), P(Csr), P(C.c_int64), P(C.c_int64)]),
     "alls_autotune": (C.c_int, [vp, P(Mesh), P(Opts), P(Opts)]),
+    "alls_csr_write_mtx": (C.c_int, [vp, P(Csr), C.c_char_p]),
+    "alls_mesh_save": (C.c_int, [P(Mesh), C.c_char_p]),
+    "alls_mesh_load": (C.c_int, [C.c_char_p, P(HostMesh)]),
     "alls_server_create": (C.c_int, [P(ServerConfig), P(vp)]),
     "alls_server_submit": (C.c_int, [vp, P(Mesh), C.c_uint64, P(Opts), P(C.c_int64)]),
     "alls_server_wait": (C.c_int, [vp, C.c_int64, P(Csr), P(RequestStats)]),
@@ -219,6 +222,24 @@ def generate(kind, nx, ny=None, nz=0, jitter=0.0, seed=0, random_diagonal=0, per
     s.E, s.E_sigma, s.nu, s.thickness = E, E_sigma, nu, thickness
     h = HostMesh()
     check(lib().alls_gen_mesh(C.byref(s), C.byref(h)))
+    return _from_host_mesh(h)
+
+
+def load_mesh(path):
+    """alls_mesh_load: a mesh saved with save_mesh (binary file)."""
+    h = HostMesh()
+    check(lib().alls_mesh_load(os.fsencode(path), C.byref(h)))
+    return _from_host_mesh(h)
+
+
+def save_mesh(mesh, path):
+    """alls_mesh_save: writes a host mesh to a binary file."""
+    m = mesh.c_mesh()
+    check(lib().alls_mesh_save(C.byref(m), os.fsencode(path)))
+
+
+def _from_host_mesh(h):
+    """Copies a library-owned alls_host_mesh into numpy arrays and frees it."""
     try:
         m = h.mesh
         dim, npe = m.dim, m.nodes_per_elem
@@ -422,6 +443,10 @@ class DeviceCsr:
     @property
     def n_rows(self):
         return self.csr.n_rows
+
+    def write_mtx(self, path):
+        """alls_csr_write_mtx: Matrix Market coordinate file."""
+        check(lib().alls_csr_write_mtx(self.ctx.handle, C.byref(self.csr), os.fsencode(path)), self.ctx)
 
     def download(self):
         """(row_ptr int64, col_ind int32, val float64) as numpy arrays."""

@@ -72,6 +72,27 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
         if (!WRITE && lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
         return;
     }
+    const int wl = threadIdx.x & 63;
+    const int gbase = wl - lane;
+    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
+    if constexpr (!WRITE) {
+        // Counting needs only the columns: a bitonic sort of 32-bit keys.
+        uint32_t c = lane < n ? (uint32_t)p.keys[entry_index(p, a, h, lane)].x : 0xffffffffu;
+#pragma unroll
+        for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t oc = __shfl_xor(c, j, G);
+                const bool up = (lane & k) == 0;
+                const bool lower = (lane & j) == 0;
+                if ((lower == up) ? (oc < c) : (oc > c)) c = oc;
+            }
+        }
+        const uint32_t pc = __shfl_up(c, 1, G);
+        const uint64_t heads = __ballot(lane < n && (lane == 0 || pc != c));
+        if (lane == 0) u[a] = __popcll(heads & gmask);
+        return;
+    }
     uint64_t key = KEY_INF;
     int32_t slot = lane;
     if (lane < n) key = pack_key(p.keys[entry_index(p, a, h, lane)]);
@@ -96,14 +117,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     const int32_t pcol = key_col(__shfl_up(key, 1, G));
     const bool head = valid && (lane == 0 || pcol != col);
     const uint64_t heads = __ballot(head);
-    const int wl = threadIdx.x & 63;
-    const int gbase = wl - lane;
-    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
     const uint64_t gheads = heads & gmask;
-    if (!WRITE) {
-        if (lane == 0) u[a] = __popcll(gheads);
-        return;
-    }
     double v[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) v[b] = 0.0;

@@ -302,6 +302,20 @@ typedef struct {
 alls_status alls_gen_mesh(const alls_gen_spec* spec, alls_host_mesh* out);
 void        alls_host_mesh_free(alls_host_mesh* m);
 
+/* ------------------------------------------------------------- files --- */
+/* Writes a CSR as a Matrix Market coordinate file (real general, 1-based,
+ * every stored entry, values printed with 17 significant digits so they
+ * read back bit-exact).  The reference writes only CSV timing tables
+ * (Perf:1139, Mix:838). */
+alls_status alls_csr_write_mtx(alls_ctx* ctx, const alls_csr* csr, const char* path);
+/* Binary mesh file: a 96-byte header (magic "ALLSMSH1", dim, nodes_per_elem,
+ * uniform_type, flags, n_nodes, n_elems, E, nu, thickness), then coords,
+ * conn, elem_type (flag 1) and E_per_elem (flag 2) as stored in memory.
+ * Save takes a host mesh; load returns one owned by the library (free with
+ * alls_host_mesh_free). */
+alls_status alls_mesh_save(const alls_mesh* mesh, const char* path);
+alls_status alls_mesh_load(const char* path, alls_host_mesh* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,3 +118,27 @@ def test_jacobian_signs_of_benchmark_meshes():
     t3b = alls.generate(alls.GEN_T3, 200, 200, jitter=0.3, seed=5, random_diagonal=1)
     frac = (element_areas_2d(t3b) <= 0).mean()
     assert frac < 1e-3
+
+
+@pytest.mark.parametrize("kind,kw", [(alls.GEN_Q4, dict(nx=7, ny=5, E_sigma=0.5)),
+                                     (alls.GEN_MIXED, dict(nx=6, ny=4, random_diagonal=1, E=3.0)),
+                                     (alls.GEN_HEX20, dict(nx=2, ny=1, nz=3, jitter=0.1))])
+def test_mesh_file_round_trip(tmp_path, kind, kw):
+    m = alls.generate(kind, seed=4, **kw)
+    path = tmp_path / "m.bin"
+    alls.save_mesh(m, path)
+    r = alls.load_mesh(path)
+    for f in ("coords", "conn", "elem_type", "E_per_elem"):
+        a, b = getattr(m, f), getattr(r, f)
+        assert (a is None and b is None) or np.array_equal(a, b), f
+    assert (r.dim, r.npe, r.uniform_type, r.E, r.nu, r.thickness) == (m.dim, m.npe, m.uniform_type, m.E, m.nu,
+                                                                      m.thickness)
+
+
+def test_mesh_file_rejects_garbage(tmp_path):
+    path = tmp_path / "bad.bin"
+    path.write_bytes(b"not a mesh file at all" * 8)
+    with pytest.raises(alls.AllsError):
+        alls.load_mesh(path)
+    with pytest.raises(alls.AllsError):
+        alls.load_mesh(tmp_path / "missing.bin")

@@ -187,3 +187,18 @@ def test_c2_q4_2000_full_size_invariants(ctx):
     expect = (element_areas_2d(m) * m.E_per_elem).sum() * (ev @ plane_stress_D(1.0, m.nu) @ ev)
     u = uniform_strain_field(X, 2, eps)
     assert abs(u @ csr_matvec(rp, ci, v, u) - expect) < 1e-9 * abs(expect)
+
+
+def test_matrix_market_file_round_trips_exactly(ctx, tmp_path):
+    import scipy.io
+    m = alls.generate(alls.GEN_MIXED, 9, 7, jitter=0.25, seed=17, random_diagonal=1, E_sigma=0.5)
+    out = ctx.assemble(m)
+    rp, ci, v = out.download()
+    path = tmp_path / "K.mtx"
+    out.write_mtx(path)
+    out.free()
+    K = scipy.io.mmread(str(path)).tocsr()
+    K.sort_indices()
+    assert K.shape == (rp.size - 1, rp.size - 1)
+    assert np.array_equal(K.indptr, rp) and np.array_equal(K.indices, ci)
+    assert np.array_equal(K.data, v)
