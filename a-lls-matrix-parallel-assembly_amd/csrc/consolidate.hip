@@ -67,17 +67,40 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     const int lane = (int)(gid % G);
     if (a >= p.n_nodes) return;
     const int32_t* h = p.hdr + a * ALLS_DIR;
+    // The home chunk (slots < C) does not depend on the header, so its key
+    // and values are loaded together with it: one memory round trip instead
+    // of header -> key -> value.  Slots past the count are loaded but unused.
+    const bool home = lane < p.C;
+    int2 kh = make_int2(0, 0);
+    if (home) kh = p.keys[a * p.C + lane];
+    VT vh[WRITE ? B : 1];
+    if constexpr (WRITE) {
+        if (home) {
+            const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + lane) * B;
+#pragma unroll
+            for (int b = 0; b < B; ++b) vh[b] = src[b];
+        }
+    }
     const int32_t n = min(h[0], ALLS_DIR * p.C);
     if (n > G) {
         if (!WRITE && lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
         return;
+    }
+    if (!home && lane < n) {   // overflow chunk (rare for rows this short)
+        const int64_t ei = entry_index(p, a, h, lane);
+        kh = p.keys[ei];
+        if constexpr (WRITE) {
+            const VT* src = static_cast<const VT*>(p.vals) + ei * B;
+#pragma unroll
+            for (int b = 0; b < B; ++b) vh[b] = src[b];
+        }
     }
     const int wl = threadIdx.x & 63;
     const int gbase = wl - lane;
     const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
     if constexpr (!WRITE) {
         // Counting needs only the columns: a bitonic sort of 32-bit keys.
-        uint32_t c = lane < n ? (uint32_t)p.keys[entry_index(p, a, h, lane)].x : 0xffffffffu;
+        uint32_t c = lane < n ? (uint32_t)kh.x : 0xffffffffu;
 #pragma unroll
         for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
@@ -95,7 +118,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     }
     uint64_t key = KEY_INF;
     int32_t slot = lane;
-    if (lane < n) key = pack_key(p.keys[entry_index(p, a, h, lane)]);
+    if (lane < n) key = pack_key(kh);
     // Bitonic sort of G keys across the group's lanes.
 #pragma unroll
     for (int k = 2; k <= G; k <<= 1) {
@@ -118,13 +141,12 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     const bool head = valid && (lane == 0 || pcol != col);
     const uint64_t heads = __ballot(head);
     const uint64_t gheads = heads & gmask;
+    // Values follow their keys: the lane that loaded slot `slot` holds them.
     double v[B];
 #pragma unroll
-    for (int b = 0; b < B; ++b) v[b] = 0.0;
-    if (valid) {
-        const VT* src = static_cast<const VT*>(p.vals) + entry_index(p, a, h, slot) * B;
-#pragma unroll
-        for (int b = 0; b < B; ++b) v[b] = (double)src[b];
+    for (int b = 0; b < B; ++b) {
+        const double x = __shfl((double)vh[b], slot, G);
+        v[b] = valid ? x : 0.0;
     }
     // Segmented inclusive scan inside runs of equal column.
 #pragma unroll
