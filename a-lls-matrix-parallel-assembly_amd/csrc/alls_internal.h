@@ -82,8 +82,7 @@ struct Scatter {
     int64_t count;
     int32_t i64;               // row_ptr is int64
     int32_t npe;               // map stride per node (4 in 2D, 20 for Hex20)
-    const int64_t* slot;       // Hex20 staged path: slot of each contribution
-    void* stage;               // Hex20 staged path: [slots][9] values (T)
+    void* stage;               // Hex20 value-only path: [n_elems*400][9] blocks in element order (T)
 };
 
 #ifdef __HIPCC__
@@ -186,15 +185,15 @@ void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic
 void launch_gather_2d(const DevMesh& m, int kind, const Scatter& sc, const int64_t* inc_off, const int32_t* inc_list,
                       int32_t* misc, hipStream_t s);
 // Hex20 scatter: mode 1 = FP64 atomics, 0 = plain "+=" (one colour),
-// 3 = write each block to its staging slot (value-only path).
+// 3 = write each block to stage[e*400 + I*20 + J] (value-only path).
 void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int mode, int f32, int32_t* misc, hipStream_t s);
-void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
-                       const int64_t* inc_off, const int32_t* inc_list, int32_t* cnt, hipStream_t s);
-void launch_slot_map(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
-                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int32_t* fill,
-                     int64_t* slot, hipStream_t s);
-void launch_pair_sum(int64_t n_nodes, const void* rp, int i64, const int64_t* goff, const void* stage, int f32,
-                     double* val, hipStream_t s);
+// Hex20 value-only path: the largest column count of a node row (atomicMax
+// into *out), and the per-row ordered gather of element-order staged blocks.
+void launch_row_umax(int64_t n_nodes, const void* rp, int i64, int32_t* out, hipStream_t s);
+int32_t pair_gather_max_cols();
+void launch_pair_gather(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
+                        const int64_t* inc_off, const int32_t* inc_list, const void* stage, int f32, int32_t umax,
+                        double* val, hipStream_t s);
 void launch_fill_f64(double* p, int64_t n, double v, hipStream_t s);
 // Colouring: node -> element incidence, JPL rounds, conflict check, buckets.
 void launch_incidence(const DevMesh& m, int32_t* cnt, int64_t* off, int32_t* cursor, int32_t* list, int64_t* tiles,

@@ -1,152 +1,127 @@
 // cached_hex20.hip -- value-only Hex20 re-assembly on a cached pattern
 // (gfx950), without atomics, keys or sorting.
 //
-// At pattern build time every contribution (element e, row node I, column
-// node J) gets a fixed slot in a staging array grouped by node pair
-// (a = conn[e][I], rank r of b = conn[e][J] in row a), elements inside a group
-// in ascending id (the serial order of Acc:336):
-//   k_pair_count : contributions per node pair (elements sharing a and b);
-//   (scan)       : group offsets;
-//   k_slot_map   : slot of each contribution.
-// Both walk each node row's incident elements with one thread, which owns
-// the row's node-pair counters, so they need no atomics.
-// A repeat request then integrates each element (k_hex20 with EMIT 3 writes
-// its 400 blocks to their slots) and k_pair_sum adds every group and writes
-// the 3x3 block into the CSR rows.
+// A repeat request integrates every element (k_hex20 with EMIT 3) and stores
+// its 400 node-pair blocks in element order, blocks (I, J) of element e at
+// stage[e*400 + I*20 + J]: contiguous, fully coalesced stores.  Then
+// k_pair_gather assembles each node row a from its incident elements: the
+// blocks (I_a, 0..19) of an element are contiguous, so every element
+// contributes one 20-block segment (1.4 KB in FP64), and the cached rank map
+// gives each block's column position.  The elements are taken in ascending
+// id (the incidence list is sorted), three at a time for the loads and one
+// after the other for the sums, so every column is summed in the serial
+// order of Acc:336, exactly as the plain path sums it.
+#include <algorithm>
+
 #include "alls_internal.h"
 
 namespace {
 
-// Local index of node a in element e (a is one of its 20 nodes).
-__device__ __forceinline__ int local_index(const int32_t* conn, int64_t e, int64_t a) {
-    const int32_t* c = conn + e * 20;
-    int I = 0;
-#pragma unroll
-    for (int i = 0; i < 20; ++i)
-        if (c[i] == a) I = i;
-    return I;
-}
-
-// Thread per node row a: the row owns its node pairs g0 .. g0+u-1, so the
-// counters need no atomics.  For every element incident to a, every one of
-// its 20 column nodes adds one contribution to pair (a, rank).
-template <typename RP>
-__global__ __launch_bounds__(256) void k_pair_count(int64_t n_nodes, const int32_t* conn, const RP* rp,
-                                                    const int32_t* rank_map, const int64_t* inc_off,
-                                                    const int32_t* inc_list, int32_t* cnt) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_nodes) return;
-    const int64_t r0 = (int64_t)rp[3 * a];
-    const int64_t u = ((int64_t)rp[3 * a + 1] - r0) / 3;
-    const int64_t g0 = r0 / 9;
-    for (int64_t r = 0; r < u; ++r) cnt[g0 + r] = 0;
-    for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) {
-        const int64_t e = inc_list[k];
-        const int32_t* mp = rank_map + e * 400 + local_index(conn, e, a) * 20;
-        for (int J = 0; J < 20; ++J) cnt[g0 + mp[J]] += 1;
-    }
-}
-
-// Thread per node row a, elements in ascending id (the incidence list is
-// sorted): contribution (e, I, J) takes the next free slot of its pair's
-// group, so groups come out in the serial order of Acc:336.  `fill` holds
-// the row's running counts (zeroed here).
-template <typename RP>
-__global__ __launch_bounds__(256) void k_slot_map(int64_t n_nodes, const int32_t* conn, const RP* rp,
-                                                  const int32_t* rank_map, const int64_t* inc_off,
-                                                  const int32_t* inc_list, const int64_t* goff, int32_t* fill,
-                                                  int64_t* slot) {
-    const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= n_nodes) return;
-    const int64_t r0 = (int64_t)rp[3 * a];
-    const int64_t u = ((int64_t)rp[3 * a + 1] - r0) / 3;
-    const int64_t g0 = r0 / 9;
-    for (int64_t r = 0; r < u; ++r) fill[g0 + r] = 0;
-    for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) {
-        const int64_t e = inc_list[k];
-        const int I = local_index(conn, e, a);
-        const int32_t* mp = rank_map + e * 400 + I * 20;
-        for (int J = 0; J < 20; ++J) {
-            const int64_t g = g0 + mp[J];
-            slot[e * 400 + I * 20 + J] = goff[g] + fill[g]++;
-        }
-    }
-}
-
-// Wave per node row: sums each node pair's contributions in slot order and
-// writes the 3x3 block to DOF rows 3a..3a+2.
+// Wave per node row; acc (LDS) holds the row's u column blocks.
 template <typename T, typename RP>
-__global__ __launch_bounds__(256) void k_pair_sum(int64_t n_nodes, const RP* rp, const int64_t* goff, const T* stage,
-                                                  double* val) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t a = wave; a < n_nodes; a += nw) {
+__global__ __launch_bounds__(64) void k_pair_gather(int64_t n_nodes, const int32_t* conn, const RP* rp,
+                                                    const int32_t* rank_map, const int64_t* inc_off,
+                                                    const int32_t* inc_list, const T* stage, double* val) {
+    extern __shared__ double acc[];   // [u][9]
+    const int lane = threadIdx.x;
+    const int sub = lane / 20, J = lane - 20 * sub;   // element of the step (0..2; 3 = idle), column node
+    for (int64_t a = xcd_block(blockIdx.x, gridDim.x); a < n_nodes; a += gridDim.x) {
         const int64_t r0 = (int64_t)rp[3 * a], r1 = (int64_t)rp[3 * a + 1], r2 = (int64_t)rp[3 * a + 2];
         const int64_t u = (r1 - r0) / 3;
-        const int64_t g0 = r0 / 9;
-        for (int64_t r = lane; r < u; r += 64) {
-            double acc[9];
-#pragma unroll
-            for (int v = 0; v < 9; ++v) acc[v] = 0.0;
-            for (int64_t s = goff[g0 + r]; s < goff[g0 + r + 1]; ++s) {
-#pragma unroll
-                for (int v = 0; v < 9; ++v) acc[v] += (double)stage[s * 9 + v];
+        for (int64_t i = lane; i < u * 9; i += 64) acc[i] = 0.0;
+        __syncthreads();
+        const int64_t k0 = inc_off[a], k1 = inc_off[a + 1];
+        for (int64_t k = k0; k < k1; k += 3) {
+            const int64_t kk = k + sub;
+            const bool active = sub < 3 && kk < k1;
+            int64_t e = 0;
+            int32_t cj = -1;
+            if (active) {
+                e = inc_list[kk];
+                cj = conn[e * 20 + J];
             }
-            const int64_t rows[3] = {r0, r1, r2};
+            // Local index of node a in each of the three elements.
+            const uint64_t hit = __ballot(active && cj == a);
+            const uint32_t mine = (uint32_t)(hit >> (20 * (sub < 3 ? sub : 0))) & 0xfffffu;
+            const int I = mine ? __ffs(mine) - 1 : -1;
+            int32_t r = -1;
+            double v[9];
+            if (active && I >= 0) {
+                const int64_t b = e * 400 + I * 20 + J;
+                r = rank_map[b];
+                const T* src = stage + b * 9;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) v[q] = (double)src[q];
+            }
+            // In element order: the three elements add one after the other.
+#pragma unroll
+            for (int s = 0; s < 3; ++s) {
+                if (sub == s && r >= 0) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) acc[r * 9 + q] += v[q];
+                }
+                __syncthreads();
+            }
+        }
+        const int64_t rows[3] = {r0, r1, r2};
+        for (int64_t c = lane; c < u; c += 64) {
 #pragma unroll
             for (int dr = 0; dr < 3; ++dr)
 #pragma unroll
-                for (int dc = 0; dc < 3; ++dc) val[rows[dr] + 3 * r + dc] = acc[dr * 3 + dc];
+                for (int dc = 0; dc < 3; ++dc) val[rows[dr] + 3 * c + dc] = acc[c * 9 + dr * 3 + dc];
         }
+        __syncthreads();
     }
 }
 
-unsigned grid_for(int64_t n) { return (unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1); }
+// Largest number of column nodes in one node row: u = (rp[3a+1] - rp[3a]) / 3.
+template <typename RP>
+__global__ __launch_bounds__(256) void k_row_umax(int64_t n_nodes, const RP* rp, int32_t* out) {
+    int32_t m = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; a < n_nodes; a += stride)
+        m = max(m, (int32_t)(((int64_t)rp[3 * a + 1] - (int64_t)rp[3 * a]) / 3));
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
+}
 
 }  // namespace
 
 namespace alls {
 
-void launch_pair_count(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
-                       const int64_t* inc_off, const int32_t* inc_list, int32_t* cnt, hipStream_t s) {
+int32_t pair_gather_max_cols() { return 900; }   // 900 x 72 B of LDS
+
+void launch_row_umax(int64_t n_nodes, const void* rp, int i64, int32_t* out, hipStream_t s) {
     if (n_nodes <= 0) return;
+    int64_t g = (n_nodes + 255) / 256;
+    if (g > 4096) g = 4096;
     if (i64)
-        hipLaunchKernelGGL(k_pair_count<int64_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
-                           static_cast<const int64_t*>(rp), rank_map, inc_off, inc_list, cnt);
+        hipLaunchKernelGGL(k_row_umax<int64_t>, dim3((unsigned)g), dim3(256), 0, s, n_nodes,
+                           static_cast<const int64_t*>(rp), out);
     else
-        hipLaunchKernelGGL(k_pair_count<int32_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
-                           static_cast<const int32_t*>(rp), rank_map, inc_off, inc_list, cnt);
+        hipLaunchKernelGGL(k_row_umax<int32_t>, dim3((unsigned)g), dim3(256), 0, s, n_nodes,
+                           static_cast<const int32_t*>(rp), out);
 }
 
-void launch_slot_map(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
-                     const int64_t* inc_off, const int32_t* inc_list, const int64_t* goff, int32_t* fill,
-                     int64_t* slot, hipStream_t s) {
+void launch_pair_gather(int64_t n_nodes, const int32_t* conn, const void* rp, int i64, const int32_t* rank_map,
+                        const int64_t* inc_off, const int32_t* inc_list, const void* stage, int f32, int32_t umax,
+                        double* val, hipStream_t s) {
     if (n_nodes <= 0) return;
-    if (i64)
-        hipLaunchKernelGGL(k_slot_map<int64_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
-                           static_cast<const int64_t*>(rp), rank_map, inc_off, inc_list, goff, fill, slot);
-    else
-        hipLaunchKernelGGL(k_slot_map<int32_t>, dim3(grid_for(n_nodes)), dim3(256), 0, s, n_nodes, conn,
-                           static_cast<const int32_t*>(rp), rank_map, inc_off, inc_list, goff, fill, slot);
-}
-
-void launch_pair_sum(int64_t n_nodes, const void* rp, int i64, const int64_t* goff, const void* stage, int f32,
-                     double* val, hipStream_t s) {
-    if (n_nodes <= 0) return;
-    int64_t blocks = (n_nodes + 3) / 4;
+    int64_t blocks = n_nodes;
     if (blocks > 262144) blocks = 262144;
-#define ALLS_PS(T, RP)                                                                                        \
-    hipLaunchKernelGGL((k_pair_sum<T, RP>), dim3((unsigned)blocks), dim3(256), 0, s, n_nodes,                 \
-                       static_cast<const RP*>(rp), goff, static_cast<const T*>(stage), val)
+    const size_t lds = (size_t)std::max(umax, 1) * 9 * sizeof(double);
+#define ALLS_PG(T, RP)                                                                                      \
+    hipLaunchKernelGGL((k_pair_gather<T, RP>), dim3((unsigned)blocks), dim3(64), lds, s, n_nodes, conn,     \
+                       static_cast<const RP*>(rp), rank_map, inc_off, inc_list, static_cast<const T*>(stage), val)
     if (f32) {
-        if (i64) ALLS_PS(float, int64_t);
-        else ALLS_PS(float, int32_t);
+        if (i64) ALLS_PG(float, int64_t);
+        else ALLS_PG(float, int32_t);
     } else {
-        if (i64) ALLS_PS(double, int64_t);
-        else ALLS_PS(double, int32_t);
+        if (i64) ALLS_PG(double, int64_t);
+        else ALLS_PG(double, int32_t);
     }
-#undef ALLS_PS
+#undef ALLS_PG
 }
 
 }  // namespace alls

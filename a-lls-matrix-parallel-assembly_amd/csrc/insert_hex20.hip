@@ -245,7 +245,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                     for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
                 p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
             } else if (EMIT == 3) {
-                T* dst = static_cast<T*>(sc.stage) + sc.slot[e * 400 + pr] * 9;
+                T* dst = static_cast<T*>(sc.stage) + (e * 400 + pr) * 9;
     #pragma unroll
                 for (int r = 0; r < 3; ++r)
     #pragma unroll

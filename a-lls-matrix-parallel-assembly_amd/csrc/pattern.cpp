@@ -29,8 +29,7 @@ struct alls_pattern {
     DevBuf row_ptr, col_ind, map;
     DevBuf inc_cnt, inc_off, inc_cursor, inc_list, tiles;   // node -> element incidence, sorted by element id
     bool have_inc = false;
-    DevBuf pair_cnt, goff, slot;                            // Hex20: contribution groups per node pair
-    int64_t n_contrib = 0;
+    int32_t umax = 0;                                       // Hex20: most column nodes in one node row
     int32_t id = 0;                                         // stamped into outputs holding the pattern
 };
 
@@ -165,32 +164,13 @@ void launch_scatter(const alls_pattern* p, const DevMesh& dm, const Scatter& sc,
     else launch_scatter_2d(dm, p->mi.kind, sc, atomic, misc, s);
 }
 
-// Hex20: contributions per node pair -> group offsets -> slot of every
-// contribution (grouped by node pair, ascending element id inside a group).
-alls_status build_hex20_slots(alls_pattern* p, hipStream_t s, std::string* err) {
-    const int64_t npairs = p->nnz / 9;
-    HIP_TRY(p->pair_cnt.ensure(std::max<int64_t>(npairs, 1) * sizeof(int32_t)), "alloc pair counts");
-    HIP_TRY(p->goff.ensure((npairs + 1) * sizeof(int64_t)), "alloc pair offsets");
-    HIP_TRY(p->slot.ensure(std::max<int64_t>(p->dm.n_elems, 1) * 400 * sizeof(int64_t)), "alloc slot map");
-    DevBuf tiles;
-    HIP_TRY(tiles.ensure((scan_tile_count(npairs) + 1) * sizeof(int64_t)), "alloc scan tiles");
-    launch_pair_count(p->dm.n_nodes, p->dm.conn, p->row_ptr.p, p->i64, p->map.as<int32_t>(),
-                      p->inc_off.as<int64_t>(), p->inc_list.as<int32_t>(), p->pair_cnt.as<int32_t>(), s);
-    launch_scan_total(p->pair_cnt.as<int32_t>(), npairs, tiles.as<int64_t>(), scan_tile_count(npairs), s);
-    launch_scan_apply(p->pair_cnt.as<int32_t>(), npairs, tiles.as<int64_t>(), 1, p->goff.p, 1, s);
-    // pair_cnt is reused as the running fill counters of the slot map.
-    launch_slot_map(p->dm.n_nodes, p->dm.conn, p->row_ptr.p, p->i64, p->map.as<int32_t>(), p->inc_off.as<int64_t>(),
-                    p->inc_list.as<int32_t>(), p->goff.as<int64_t>(), p->pair_cnt.as<int32_t>(),
-                    p->slot.as<int64_t>(), s);
-    HIP_TRY(hipGetLastError(), "launch slot map");
-    HIP_TRY(hipMemcpyAsync(&p->n_contrib, p->goff.as<int64_t>() + npairs, sizeof(int64_t), hipMemcpyDeviceToHost, s),
-            "read contributions");
-    HIP_TRY(hipStreamSynchronize(s), "sync slot map");
-    p->pair_cnt.release();
-    if (p->n_contrib != p->dm.n_elems * 400) {
-        if (err) *err = "Hex20 contribution groups do not cover every element block";
-        return ALLS_E_HIP;
-    }
+// Hex20: the largest row (column nodes) sizes the ordered gather's LDS.
+alls_status hex20_row_limit(alls_pattern* p, int32_t* misc, hipStream_t s, std::string* err) {
+    launch_fill_i32(misc, MISC_SIZE, 0, s);
+    launch_row_umax(p->dm.n_nodes, p->row_ptr.p, p->i64, misc, s);
+    HIP_TRY(hipGetLastError(), "launch row size");
+    HIP_TRY(hipMemcpyAsync(&p->umax, misc, sizeof(int32_t), hipMemcpyDeviceToHost, s), "read row size");
+    HIP_TRY(hipStreamSynchronize(s), "sync row size");
     return ALLS_OK;
 }
 
@@ -262,7 +242,7 @@ alls_status pattern_build(Workspace& ws, const alls_mesh* mesh, const alls_opts&
         return fail(ALLS_E_HIP);
     }
     if ((rc = build_incidence(p, s, err)) != ALLS_OK) return fail(rc);
-    if (mi.kind == 3 && (rc = build_hex20_slots(p, s, err)) != ALLS_OK) return fail(rc);
+    if (mi.kind == 3 && (rc = hex20_row_limit(p, ws.misc.as<int32_t>(), s, err)) != ALLS_OK) return fail(rc);
     if (hipStreamSynchronize(s) != hipSuccess) return fail(hip_fail(hipGetLastError(), "sync pattern", err));
     *out_pat = p;
     return ALLS_OK;
@@ -278,20 +258,23 @@ alls_status pattern_values(Workspace& ws, alls_pattern* p, const double* E_dev, 
     DevMesh dm = p->dm;
     dm.E_elem = E_dev;
     dm.E = E_scalar;
+    if (p->mi.kind == 3 && p->umax > pair_gather_max_cols())   // rows too wide for the gather's LDS
+        return run_alls(ws, dm, p->mi, p->opts, out, s, st, err);
     Scatter sc = make_scatter(p, out);
     sc.count = dm.n_elems;
     HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
     launch_fill_i32(ws.misc.as<int32_t>(), MISC_SIZE, 0, s);
     if (p->mi.kind == 3) {
-        // Integrate into the staging slots, then sum each node pair's group.
-        HIP_TRY(ws.vals.ensure((size_t)std::max<int64_t>(p->n_contrib, 1) * 9 * (p->mi.f32 ? 4 : 8)), "alloc stage");
-        sc.slot = p->slot.as<int64_t>();
+        // Integrate into element-order staging, then gather each node row in
+        // element order.
+        HIP_TRY(ws.vals.ensure((size_t)std::max<int64_t>(dm.n_elems, 1) * 400 * 9 * (p->mi.f32 ? 4 : 8)),
+                "alloc stage");
         sc.stage = ws.vals.p;
         HIP_TRY(hipEventRecord(ws.ev[1], s), "event");
         launch_scatter_hex20(dm, sc, 3, p->mi.f32 ? 1 : 0, ws.misc.as<int32_t>(), s);
         HIP_TRY(hipEventRecord(ws.ev[2], s), "event");
-        launch_pair_sum(dm.n_nodes, p->row_ptr.p, p->i64, p->goff.as<int64_t>(), ws.vals.p, p->mi.f32 ? 1 : 0,
-                        out->val, s);
+        launch_pair_gather(dm.n_nodes, dm.conn, p->row_ptr.p, p->i64, p->map.as<int32_t>(), p->inc_off.as<int64_t>(),
+                           p->inc_list.as<int32_t>(), ws.vals.p, p->mi.f32 ? 1 : 0, p->umax, out->val, s);
     } else {
         HIP_TRY(hipEventRecord(ws.ev[1], s), "event");
         HIP_TRY(hipEventRecord(ws.ev[2], s), "event");

@@ -87,7 +87,6 @@ int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pat
         const int64_t npm = mi.kind == 3 ? 20 : 4;
         pat = inputs + (n_rows + 1) * 8 + nnz_bound * 4 + m->n_elems * npm * npm * 4 + m->n_nodes * 24 +
               m->n_elems * mi.npe_max * 4;
-        if (mi.kind == 3) pat += (nnz_bound / 9 + 1) * 12 + m->n_elems * 400 * 8;
     }
     if (pattern_bytes) *pattern_bytes = pat;
     if (scratch_out) *scratch_out = scratch + inputs;
