@@ -12,8 +12,8 @@
 // Building a pattern costs more than a plain assembly and a cached repeat
 // saves only part of one (A-LLS itself is close to the value-only path), so
 // by default a pattern is built only when the key's history pays for it:
-// with c earlier requests of the key taken as the expected number of future
-// repeats, build when c * (plain - hit) > build - plain, using per-element
+// with max(c, 1) expected future repeats for a key seen c times before,
+// build when repeats * (plain - hit) > build - plain, using per-element
 // costs of each path measured by the server itself (running averages per
 // dimension, seeded with typical MI355X figures).
 #include <hip/hip_runtime.h>
@@ -157,7 +157,8 @@ struct alls_server {
     bool worth_building(const Request& q, int prior) const {
         if (cfg.cache_policy == ALLS_CACHE_EAGER) return true;
         const double* c = cost[q.mesh.dim == 3 ? 1 : 0];
-        return prior * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
+        const int repeats = prior > 1 ? prior : 1;   // a keyed request is expected to repeat at least once
+        return repeats * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
     }
     // Small plain requests queued right behind a plain head join it in one
     // packed batch (one pipeline, persistent insertion, split afterwards).
