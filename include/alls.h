@@ -163,10 +163,11 @@ alls_status alls_pattern_create(alls_ctx* ctx, const alls_mesh* mesh, const alls
 /* Value-only re-assembly on a cached pattern with a new per-element E field
  * (E_per_elem NULL => the E the pattern was built with).  No lists and no
  * atomics: in 2D one thread per node row recomputes its incident elements
- * and writes the row; for Hex20 every element block goes to a precomputed
- * slot and each node pair's slots are summed.  Summation runs in ascending
- * element id, so repeated calls give bitwise-equal values.  row_ptr/col_ind
- * are copied into `out` only when it does not already hold this pattern. */
+ * and writes the row; for Hex20 the element blocks are staged in element
+ * order and one wave per node row gathers its incident elements' blocks.
+ * Summation runs in ascending element id, so repeated calls give values
+ * bitwise equal to each other and to alls_assemble.  row_ptr/col_ind are
+ * copied into `out` only when it does not already hold this pattern. */
 alls_status alls_assemble_values(alls_ctx* ctx, const alls_pattern* pat, const double* E_per_elem,
                                  int32_t mem, alls_csr* out);
 int64_t     alls_pattern_nnz(const alls_pattern* pat);
