@@ -148,21 +148,22 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
         const double x = __shfl((double)vh[b], slot, G);
         v[b] = valid ? x : 0.0;
     }
-    // Segmented inclusive scan inside runs of equal column.
+    // The head lane of each run of equal column adds the run's values one by
+    // one, in sorted (element id) order: 0 + v1 + v2 + ..., the serial order,
+    // so this path gives the same bits as every other path of the library.
+    const uint64_t above = wl == 63 ? 0ull : (gheads & (~0ull << (wl + 1)));
+    const int32_t run = head ? (above ? (__ffsll((unsigned long long)above) - 1 - gbase) : n) - lane : 0;
+    for (int d = 1; d < G; ++d) {
+        if (!__any(d < run)) break;
+        double x[B];
 #pragma unroll
-    for (int d = 1; d < G; d <<= 1) {
-        const int32_t ocol = __shfl_up(col, d, G);
-        double ov[B];
+        for (int b = 0; b < B; ++b) x[b] = __shfl(v[b], lane + d, G);
+        if (d < run) {
 #pragma unroll
-        for (int b = 0; b < B; ++b) ov[b] = __shfl_up(v[b], d, G);
-        if (lane >= d && ocol == col) {
-#pragma unroll
-            for (int b = 0; b < B; ++b) v[b] += ov[b];
+            for (int b = 0; b < B; ++b) v[b] += x[b];
         }
     }
-    const int32_t ncol = __shfl_down(col, 1, G);
-    const bool tail = valid && (lane == n - 1 || ncol != col);
-    if (tail) {
+    if (head) {
         const uint64_t le = wl == 63 ? ~0ull : ((1ull << (wl + 1)) - 1);
         const int32_t rank = __popcll(gheads & le) - 1;
         if constexpr (B == 4) {

@@ -24,8 +24,12 @@ struct Elem2D {
     double Sxx[4][4], Syy[4][4], Sxy[4][4];   // Sxx/Syy upper triangles only
 
     // Loads element e; false (and an error flag) when a node id is out of range.
+    // No FMA contraction here: every kernel that integrates an element (list
+    // insertion, value-only gather, coloured scatter) then produces the same
+    // bits, so the paths agree bitwise, not only within rounding.
     template <int MODE>
     __device__ __forceinline__ bool load(const DevMesh& m, int64_t e, int32_t* misc) {
+#pragma clang fp contract(off)
         const int kind = MODE == 2 ? (int)m.types[e] : MODE;
         if (kind > 1) {
             st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
@@ -129,6 +133,7 @@ struct Elem2D {
 
     // 2x2 block of node pair (i, j): k[0..3] = K00, K01, K10, K11.
     __device__ __forceinline__ void block(int i, int j, double* k) const {
+#pragma clang fp contract(off)
         const int lo = i < j ? i : j, hi = i < j ? j : i;
         const double sxx = Sxx[lo][hi], syy = Syy[lo][hi];
         const double sxy = Sxy[i][j], syx = Sxy[j][i];

@@ -30,8 +30,14 @@ def test_value_only_reassembly_with_new_E(ctx, name, kind, kw):
     for _ in range(2):
         E2 = np.exp(0.5 * rng.standard_normal(m.n_elems))
         out = pat.values(E2)
-        compare_csr(out.download(), oracle.assemble(m.with_E(E2)))
+        got = out.download()
+        compare_csr(got, oracle.assemble(m.with_E(E2)))
         out.free()
+        # Same blocks, same element-order sums: bitwise the plain assembly.
+        plain = ctx.assemble(m.with_E(E2))
+        for a, b in zip(got, plain.download()):
+            assert np.array_equal(a, b)
+        plain.free()
     pat.free()
 
 
@@ -82,3 +88,4 @@ def test_hex20_f32_value_only(ctx):
     v = pat.values(E2).download()[2]
     ref = oracle.assemble(m.with_E(E2))[2]
     assert np.linalg.norm(v - ref) / np.linalg.norm(ref) < 1e-6
+    assert np.array_equal(v, ctx.assemble(m.with_E(E2), precision=alls.PREC_F32_MFMA).download()[2])
