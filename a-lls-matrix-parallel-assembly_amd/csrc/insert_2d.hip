@@ -200,12 +200,12 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
 // Rank of every (row node i, column node j) pair of element e in the cached
 // pattern, by binary search of column b*dpn in DOF row a*dpn (the search of
 // the reference's binary_search, Acc:214-222, done once per pattern).
-template <typename RP>
-__global__ __launch_bounds__(256) void k_map(DevMesh m, int npe_map, const RP* rp, const int32_t* ci, int32_t* map,
+template <int NN, typename RP>
+__global__ __launch_bounds__(256) void k_map(DevMesh m, const RP* rp, const int32_t* ci, int32_t* map,
                                              int32_t* misc) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t e = t / npe_map;
-    const int i = (int)(t % npe_map);
+    const int64_t e = t / NN;
+    const int i = (int)(t % NN);
     if (e >= m.n_elems) return;
     const int kind = m.types ? (int)m.types[e] : m.uniform_type;
     const int nn = kind == 0 ? 4 : kind == 1 ? 3 : 20;
@@ -213,22 +213,41 @@ __global__ __launch_bounds__(256) void k_map(DevMesh m, int npe_map, const RP* r
     const int dpn = m.dim;
     const int32_t* c = m.conn + e * m.npe;
     const int64_t a = c[i];
-    const int64_t rs = (int64_t)rp[a * dpn], re = (int64_t)rp[a * dpn + 1];
-    for (int j = 0; j < nn; ++j) {
-        const int32_t target = c[j] * dpn;
-        int64_t lo = rs, hi = re - 1, pos = -1;
-        while (lo <= hi) {
-            const int64_t mid = lo + (hi - lo) / 2;
-            const int32_t v = ci[mid];
-            if (v == target) { pos = mid; break; }
-            if (v < target) lo = mid + 1;
-            else hi = mid - 1;
+    const int64_t rs = (int64_t)rp[a * dpn];
+    const int32_t u = (int32_t)(((int64_t)rp[a * dpn + 1] - rs) / dpn);   // column nodes of row a
+    // The NN binary searches advance together, one probe each per round, so
+    // their loads are in flight at the same time instead of one after another.
+    int32_t tgt[NN], lo[NN], hi[NN], res[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+        tgt[j] = j < nn ? c[j] * dpn : -1;
+        lo[j] = 0;
+        hi[j] = j < nn ? u - 1 : -1;
+        res[j] = -1;
+    }
+    for (int round = 0; round < 32; ++round) {
+        bool open = false;
+#pragma unroll
+        for (int j = 0; j < NN; ++j) {
+            if (res[j] < 0 && lo[j] <= hi[j]) {
+                const int32_t mid = (lo[j] + hi[j]) >> 1;
+                const int32_t v = ci[rs + (int64_t)mid * dpn];
+                if (v == tgt[j]) res[j] = mid;
+                else if (v < tgt[j]) lo[j] = mid + 1;
+                else hi[j] = mid - 1;
+                open |= res[j] < 0 && lo[j] <= hi[j];
+            }
         }
-        if (pos < 0) {
+        if (!open) break;
+    }
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+        if (j >= nn) continue;
+        if (res[j] < 0) {
             st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
-            pos = rs;
+            res[j] = 0;
         }
-        map[e * npe_map * npe_map + i * npe_map + j] = (int32_t)((pos - rs) / dpn);
+        map[e * NN * NN + i * NN + j] = res[j];
     }
 }
 
@@ -287,12 +306,17 @@ void launch_build_map(const DevMesh& m, int kind, const void* row_ptr, int i64, 
     if (m.n_elems <= 0) return;
     const int npe_map = kind == 3 ? 20 : 4;
     const unsigned g = grid_for(m.n_elems * npe_map, 256);
-    if (i64)
-        hipLaunchKernelGGL(k_map<int64_t>, dim3(g), dim3(256), 0, s, m, npe_map,
-                           static_cast<const int64_t*>(row_ptr), col_ind, map, misc);
-    else
-        hipLaunchKernelGGL(k_map<int32_t>, dim3(g), dim3(256), 0, s, m, npe_map,
-                           static_cast<const int32_t*>(row_ptr), col_ind, map, misc);
+#define ALLS_MAP(NN, RP)                                                                                \
+    hipLaunchKernelGGL((k_map<NN, RP>), dim3(g), dim3(256), 0, s, m, static_cast<const RP*>(row_ptr), \
+                       col_ind, map, misc)
+    if (npe_map == 20) {
+        if (i64) ALLS_MAP(20, int64_t);
+        else ALLS_MAP(20, int32_t);
+    } else {
+        if (i64) ALLS_MAP(4, int64_t);
+        else ALLS_MAP(4, int32_t);
+    }
+#undef ALLS_MAP
 }
 
 void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic, int32_t* misc, hipStream_t s) {

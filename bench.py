@@ -147,10 +147,13 @@ def time_calls(fn, steps, warmup):
 def coloring_compare(ctx, mesh, dm, steps, alls_ms):
     """Cached-pattern value-only (per-row gather, no atomics) vs GPU colouring on the same mesh."""
     import alls
+    alls.Pattern(ctx, mesh, c_mesh=dm.c_mesh()).free()   # warm-up
+    t0 = time.perf_counter()
     pat = alls.Pattern(ctx, mesh, c_mesh=dm.c_mesh())
+    t_build = time.perf_counter() - t0
     pat.out.free()
     Ep = dm.E_per_elem
-    res = {}
+    res = {"pattern_build_ms": round(t_build * 1e3, 3)}
     dt, out = time_calls(lambda o: pat.values(device_ptr=Ep, out=o), steps, 2)
     res["value_only_ms"] = round(dt * 1e3, 4)
     st = ctx.stats()
@@ -197,7 +200,9 @@ def hex20_compare(ctx, n, steps):
     # Cached pattern: value-only re-assembly (staged, no atomics) and the
     # coloured path with the 8-colour cell-parity colouring.
     for name, prec in (("f64", alls.PREC_F64), ("f32_mfma", alls.PREC_F32_MFMA)):
+        t0 = time.perf_counter()
         pat = alls.Pattern(ctx, m, precision=prec, c_mesh=dm.c_mesh())
+        res[name]["pattern_build_ms"] = round((time.perf_counter() - t0) * 1e3, 3)
         pat.out.free()
         dt, out = time_calls(lambda o: pat.values(out=o), steps, 1)
         res[name]["value_only_ms"] = round(dt * 1e3, 3)
