@@ -472,13 +472,8 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
     const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
     for (int64_t row = xcd_block(blockIdx.x, gridDim.x); row < count; row += gridDim.x) {
         const int64_t a = list ? (int64_t)list[row] : row;
-        RowDir dir;
-        dir.load(p.hdr + a * ALLS_DIR);
-        const int32_t n = min(dir.d[0], ALLS_DIR * p.C);
-        if (n > CAP) {   // the host picks EPL from the count pass; never taken
-            if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], n);
-            continue;
-        }
+        // Home-chunk slots (< C) are loaded before the header arrives: they
+        // do not depend on it.  Slots past the count are loaded but unused.
         int2 kk[EPL];
         VT vv[EPL][B];
 #pragma unroll
@@ -487,7 +482,25 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
             kk[k] = make_int2(0, 0);
 #pragma unroll
             for (int b = 0; b < B; ++b) vv[k][b] = VT(0);
-            if (i < n) {
+            if (i < p.C) {
+                const int64_t ei = a * p.C + i;
+                kk[k] = p.keys[ei];
+                const VT* src = static_cast<const VT*>(p.vals) + ei * B;
+#pragma unroll
+                for (int b = 0; b < B; ++b) vv[k][b] = src[b];
+            }
+        }
+        RowDir dir;
+        dir.load(p.hdr + a * ALLS_DIR);
+        const int32_t n = min(dir.d[0], ALLS_DIR * p.C);
+        if (n > CAP) {   // the host picks EPL from the count pass; never taken
+            if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], n);
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            if (i >= p.C && i < n) {   // overflow chunks
                 const int64_t ei = dir.entry(p, a, i);
                 kk[k] = p.keys[ei];
                 const VT* src = static_cast<const VT*>(p.vals) + ei * B;
@@ -647,28 +660,36 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
     extern __shared__ int32_t table[];
     const int lane = threadIdx.x;
     const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
-    const int shift = 32 - (31 - __clz(ts));
     int32_t longest = 0;   // this block's longest row; one atomic per block
     for (int64_t r = blockIdx.x; r < count; r += gridDim.x) {
         const int64_t a = list ? (int64_t)list[r] : r;
+        // The first two keys of this lane in the home chunk are loaded
+        // together with the row header (they do not depend on it).
+        int32_t hk0 = 0, hk1 = 0;
+        if (lane < p.C) hk0 = p.keys[a * p.C + lane].x;
+        if (lane + 64 < p.C) hk1 = p.keys[a * p.C + lane + 64].x;
         RowDir dir;
         dir.load(p.hdr + a * ALLS_DIR);
         const int32_t n = min(dir.d[0], ALLS_DIR * p.C);
         longest = max(longest, n);
-        for (int i = lane; i < ts; i += 64) table[i] = -1;
+        // A table of at least twice the row's entries (power of two, <= ts).
+        int tsr = 128;
+        while (tsr < 2 * n && tsr < ts) tsr <<= 1;
+        const int shift = 32 - (31 - __clz(tsr));
+        for (int i = lane; i < tsr; i += 64) table[i] = -1;
         __syncthreads();
         int32_t mine = 0;
         for (int i = lane; i < n; i += 64) {
-            const int32_t col = p.keys[dir.entry(p, a, i)].x;
+            const int32_t col = i < p.C && i < lane + 128 ? (i == lane ? hk0 : hk1) : p.keys[dir.entry(p, a, i)].x;
             uint32_t slot = col_hash(col, shift);
-            for (int probe = 0; probe < ts; ++probe) {
+            for (int probe = 0; probe < tsr; ++probe) {
                 const int32_t old = atomicCAS(&table[slot], -1, col);
                 if (old == -1) {
                     ++mine;
                     break;
                 }
                 if (old == col) break;
-                slot = (slot + 1) & (uint32_t)(ts - 1);
+                slot = (slot + 1) & (uint32_t)(tsr - 1);
             }
         }
         int32_t total;
