@@ -4,19 +4,21 @@
 // (Acc:605-645) and its convertALLStoCSRKernel (Acc:471-501) walks each row's
 // list with one thread, binary-searching every node's column in the host
 // pattern.  Here the pattern itself is produced on the device:
-//   count : per node row, sort its (column, element) keys and count unique
-//           columns u[a];
+//   count : per node row, count its unique columns u[a];
 //   scan  : u -> row_ptr (tile sums, one-block scan of the tile sums, apply);
-//   write : sort again, merge equal columns (sum in element-id order) and
-//           write col_ind / val of the dpn DOF rows of the node.
+//   write : merge equal columns (sum in element-id order) and write
+//           col_ind / val of the dpn DOF rows of the node.
 // Rows of <= G entries use G lanes and a register bitonic sort through
-// cross-lane shuffles.  Longer rows (Hex20, or unstructured outliers) use a
-// wave per row and LDS: a hash set of the columns for the count; for the
-// write, the same hash set, column ranks by counting, a bucket per column and
-// an element-id sort inside each bucket (a handful of entries), instead of a
-// full sort of the row.  Either way every column's contributions are summed
-// in ascending element id, so the values do not depend on the order in which
-// atomics appended the entries.
+// cross-lane shuffles (32-bit columns to count; (column, element) keys to
+// write, after which the head lane of each run adds the run in order).
+// Longer rows (Hex20, or unstructured outliers) use a wave per row and LDS: a
+// hash set of the columns for the count; for the write, the same hash set,
+// column ranks by counting, a bucket per column and an element-id sort inside
+// each bucket (a handful of entries), instead of a full sort of the row; rows
+// beyond 1024 entries use an LDS bitonic sort (k_rows_big).  Either way every
+// column's contributions are summed one by one in ascending element id from
+// 0, so the values do not depend on the order in which atomics appended the
+// entries.
 #include "alls_internal.h"
 
 namespace {

@@ -4,9 +4,12 @@
 // The reference rebuilds its host pattern (Acc:605-645) for every run and
 // its only value kernels are the A-LLS pair and the coloured kernels, which
 // binary-search every scalar column (Acc:556-566).  A pattern object here
-// keeps, on the device: the mesh geometry, row_ptr/col_ind, and an element ->
-// CSR-slot map built once by binary search, so a request with a new E field
-// only integrates and scatters (FP64 atomics, or one plain launch per colour).
+// keeps, on the device: the mesh geometry, row_ptr/col_ind, an element ->
+// column-rank map built once by binary search and the node -> element
+// incidence sorted by element id.  A request with a new E field then only
+// integrates and gathers each row from its incident elements in element
+// order (no lists, no atomics: k_gather_2d, or k_hex20 + k_pair_gather), or,
+// for the coloured comparison, scatters with one plain launch per colour.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
