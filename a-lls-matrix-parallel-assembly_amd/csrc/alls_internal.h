@@ -24,6 +24,7 @@ enum {
     ERR_INVALID_MESH = 2,
     ERR_SPIN_TIMEOUT = 3,
     ERR_ROW_TOO_BIG = 4,
+    ERR_LOOKBACK = 5,          // fused pass: a look-back wait hit its spin limit
     ERR_SLOTS = 8
 };
 // misc[] layout (int32 each, one buffer so a single copy brings them back).
@@ -32,6 +33,10 @@ enum {
     MISC_OVF_TOP = 8,          // overflow chunks taken
     MISC_BIG_COUNT = 9,        // rows routed to the wave-per-row kernel
     MISC_MAX_BIG = 10,         // most entries in one of those rows
+    MISC_MAX_ROW = 11,         // longest row seen by the insertion, when above ListPool::long_row
+    MISC_TICKET = 12,          // tile tickets of the fused small-row pass
+    MISC_TOTAL_LO = 13,        // unique node pairs found by the fused pass (low 31 bits)
+    MISC_TOTAL_HI = 14,        //   and the bits above
     MISC_SIZE = 16
 };
 
@@ -59,6 +64,7 @@ struct ListPool {
     int32_t C;                 // entries per chunk (<= ALLS_MAX_CHUNK)
     int32_t B;                 // values per entry
     uint32_t cmagic;           // chunk_magic(C): s / C as a multiply-high
+    int32_t long_row;          // reservations ending past this many entries record MISC_MAX_ROW
 };
 
 #define ALLS_MAX_CHUNK 16384
@@ -117,7 +123,9 @@ ALLS_DEV int64_t xcd_block(int64_t bid, int64_t nb) {
 ALLS_DEV int32_t chunk_of(const ListPool& p, int32_t s) { return (int32_t)__umulhi((uint32_t)s, p.cmagic); }
 
 ALLS_DEV int32_t list_reserve(const ListPool& p, int32_t a, int32_t n) {
-    return add_agent(&p.hdr[(int64_t)a * ALLS_DIR], n);
+    const int32_t pos = add_agent(&p.hdr[(int64_t)a * ALLS_DIR], n);
+    if (pos + n > p.long_row) max_agent(&p.misc[MISC_MAX_ROW], pos + n);
+    return pos;
 }
 
 // The reservation [pos, pos+n) owns every chunk k >= 1 whose first slot
@@ -243,6 +251,11 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
 void launch_write(const ListPool& p, int small_group, int f32vals, int dpn, int32_t max_big, const int32_t* big_list,
                   const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s);
 int64_t scan_tile_count(int64_t n);
+// Fused count + scan + write of 2D meshes whose rows all hold <= G entries
+// (tstat: fused_tile_count int64 tile states, zeroed; misc[MISC_TICKET] zero).
+int64_t fused_tile_count(int64_t n_nodes, int G);
+void launch_rows_fused(const ListPool& p, int G, int64_t* tstat, void* row_ptr, int i64, int32_t* col_ind,
+                       double* val, hipStream_t s);
 // Most list entries one node row may hold (consolidation LDS limit).
 int32_t long_row_limit();
 

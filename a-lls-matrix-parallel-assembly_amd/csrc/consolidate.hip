@@ -60,68 +60,46 @@ struct RowDir {
 };
 
 // ------------------------------------------------ register (small) rows ---
-template <int G, int B, typename VT, bool WRITE, typename RP>
-__global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int32_t* big_list,
-                                                    const RP* row_ptr, int32_t* col_ind, double* val,
-                                                    int dpn) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t a = gid / G;
-    const int lane = (int)(gid % G);
-    if (a >= p.n_nodes) return;
+// One node row of at most G entries, merged in registers by its G lanes:
+// the keys and values are loaded (home chunk together with the header),
+// sorted by (column, element) with a bitonic network of shuffles, and the
+// head lane of each run of equal column adds the run one by one in element
+// order (0 + v1 + v2 + ..., the serial order, so this path gives the same
+// bits as every other path of the library).  Returns false for a longer row.
+template <int G, int B, typename VT>
+struct SmallRow {
+    int32_t col, rank, u;
+    bool head;
+    double v[B];
+};
+
+template <int G, int B, typename VT>
+__device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, int lane, SmallRow<G, B, VT>& r) {
     const int32_t* h = p.hdr + a * ALLS_DIR;
-    // The home chunk (slots < C) does not depend on the header, so its key
-    // and values are loaded together with it: one memory round trip instead
-    // of header -> key -> value.  Slots past the count are loaded but unused.
     const bool home = lane < p.C;
     int2 kh = make_int2(0, 0);
-    if (home) kh = p.keys[a * p.C + lane];
-    VT vh[WRITE ? B : 1];
-    if constexpr (WRITE) {
-        if (home) {
-            const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + lane) * B;
+    VT vh[B];
+    if (home) {
+        kh = p.keys[a * p.C + lane];
+        const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + lane) * B;
 #pragma unroll
-            for (int b = 0; b < B; ++b) vh[b] = src[b];
-        }
+        for (int b = 0; b < B; ++b) vh[b] = src[b];
     }
     const int32_t n = min(h[0], ALLS_DIR * p.C);
-    if (n > G) {
-        if (!WRITE && lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
-        return;
-    }
+    if (n > G) return false;
     if (!home && lane < n) {   // overflow chunk (rare for rows this short)
         const int64_t ei = entry_index(p, a, h, lane);
         kh = p.keys[ei];
-        if constexpr (WRITE) {
-            const VT* src = static_cast<const VT*>(p.vals) + ei * B;
+        const VT* src = static_cast<const VT*>(p.vals) + ei * B;
 #pragma unroll
-            for (int b = 0; b < B; ++b) vh[b] = src[b];
-        }
+        for (int b = 0; b < B; ++b) vh[b] = src[b];
     }
     const int wl = threadIdx.x & 63;
     const int gbase = wl - lane;
     const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
-    if constexpr (!WRITE) {
-        // Counting needs only the columns: a bitonic sort of 32-bit keys.
-        uint32_t c = lane < n ? (uint32_t)kh.x : 0xffffffffu;
-#pragma unroll
-        for (int k = 2; k <= G; k <<= 1) {
-#pragma unroll
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                const uint32_t oc = __shfl_xor(c, j, G);
-                const bool up = (lane & k) == 0;
-                const bool lower = (lane & j) == 0;
-                if ((lower == up) ? (oc < c) : (oc > c)) c = oc;
-            }
-        }
-        const uint32_t pc = __shfl_up(c, 1, G);
-        const uint64_t heads = __ballot(lane < n && (lane == 0 || pc != c));
-        if (lane == 0) u[a] = __popcll(heads & gmask);
-        return;
-    }
     uint64_t key = KEY_INF;
     int32_t slot = lane;
     if (lane < n) key = pack_key(kh);
-    // Bitonic sort of G keys across the group's lanes.
 #pragma unroll
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
@@ -137,55 +115,224 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             }
         }
     }
-    const int32_t col = key_col(key);
+    r.col = key_col(key);
     const bool valid = lane < n;
     const int32_t pcol = key_col(__shfl_up(key, 1, G));
-    const bool head = valid && (lane == 0 || pcol != col);
-    const uint64_t heads = __ballot(head);
-    const uint64_t gheads = heads & gmask;
-    // Values follow their keys: the lane that loaded slot `slot` holds them.
-    double v[B];
+    r.head = valid && (lane == 0 || pcol != r.col);
+    const uint64_t gheads = __ballot(r.head) & gmask;
+    r.u = __popcll(gheads);
+    const uint64_t le = wl == 63 ? ~0ull : ((1ull << (wl + 1)) - 1);
+    r.rank = __popcll(gheads & le) - 1;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        const double x = __shfl((double)vh[b], slot, G);
-        v[b] = valid ? x : 0.0;
+        const double x = __shfl((double)vh[b], slot, G);   // values follow their keys
+        r.v[b] = valid ? x : 0.0;
     }
-    // The head lane of each run of equal column adds the run's values one by
-    // one, in sorted (element id) order: 0 + v1 + v2 + ..., the serial order,
-    // so this path gives the same bits as every other path of the library.
     const uint64_t above = wl == 63 ? 0ull : (gheads & (~0ull << (wl + 1)));
-    const int32_t run = head ? (above ? (__ffsll((unsigned long long)above) - 1 - gbase) : n) - lane : 0;
+    const int32_t run = r.head ? (above ? (__ffsll((unsigned long long)above) - 1 - gbase) : n) - lane : 0;
     for (int d = 1; d < G; ++d) {
         if (!__any(d < run)) break;
         double x[B];
 #pragma unroll
-        for (int b = 0; b < B; ++b) x[b] = __shfl(v[b], lane + d, G);
+        for (int b = 0; b < B; ++b) x[b] = __shfl(r.v[b], lane + d, G);
         if (d < run) {
 #pragma unroll
-            for (int b = 0; b < B; ++b) v[b] += x[b];
+            for (int b = 0; b < B; ++b) r.v[b] += x[b];
         }
     }
-    if (head) {
-        const uint64_t le = wl == 63 ? ~0ull : ((1ull << (wl + 1)) - 1);
-        const int32_t rank = __popcll(gheads & le) - 1;
-        if constexpr (B == 4) {
-            // 2 DOF per node: row_ptr values are even, so each block row is one
-            // aligned int2 + double2 store.
+    return true;
+}
+
+// Head lanes write their dpn x dpn block at rank `rank` of the node's rows,
+// whose first entries are rs[dr].
+template <int B>
+__device__ __forceinline__ void small_row_store(int32_t col, int32_t rank, const double* v, const int64_t* rs,
+                                                int32_t* col_ind, double* val, int dpn) {
+    if constexpr (B == 4) {
+        // 2 DOF per node: row starts are even, so each block row is one
+        // aligned int2 + double2 store.
 #pragma unroll
-            for (int dr = 0; dr < 2; ++dr) {
-                const int64_t rs = (int64_t)row_ptr[a * 2 + dr] + (int64_t)rank * 2;
-                *reinterpret_cast<int2*>(col_ind + rs) = make_int2(col * 2, col * 2 + 1);
-                *reinterpret_cast<double2*>(val + rs) = make_double2(v[dr * 2], v[dr * 2 + 1]);
-            }
-        } else {
-            for (int dr = 0; dr < dpn; ++dr) {
-                const int64_t rs = (int64_t)row_ptr[a * dpn + dr] + (int64_t)rank * dpn;
-                for (int dc = 0; dc < dpn; ++dc) {
-                    col_ind[rs + dc] = col * dpn + dc;
-                    val[rs + dc] = v[dr * dpn + dc];
-                }
+        for (int dr = 0; dr < 2; ++dr) {
+            const int64_t q = rs[dr] + (int64_t)rank * 2;
+            *reinterpret_cast<int2*>(col_ind + q) = make_int2(col * 2, col * 2 + 1);
+            *reinterpret_cast<double2*>(val + q) = make_double2(v[dr * 2], v[dr * 2 + 1]);
+        }
+    } else {
+        for (int dr = 0; dr < dpn; ++dr) {
+            const int64_t q = rs[dr] + (int64_t)rank * dpn;
+            for (int dc = 0; dc < dpn; ++dc) {
+                col_ind[q + dc] = col * dpn + dc;
+                val[q + dc] = v[dr * dpn + dc];
             }
         }
+    }
+}
+
+template <int G, int B, typename VT, bool WRITE, typename RP>
+__global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int32_t* big_list,
+                                                    const RP* row_ptr, int32_t* col_ind, double* val,
+                                                    int dpn) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t a = gid / G;
+    const int lane = (int)(gid % G);
+    if (a >= p.n_nodes) return;
+    if constexpr (!WRITE) {
+        // Counting needs only the columns: a bitonic sort of 32-bit keys.
+        const int32_t* h = p.hdr + a * ALLS_DIR;
+        int32_t kc = lane < p.C ? p.keys[a * p.C + lane].x : 0;   // with the header, not after it
+        const int32_t n = min(h[0], ALLS_DIR * p.C);
+        if (n > G) {
+            if (lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
+            return;
+        }
+        if (lane >= p.C && lane < n) kc = p.keys[entry_index(p, a, h, lane)].x;
+        uint32_t c = lane < n ? (uint32_t)kc : 0xffffffffu;
+#pragma unroll
+        for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t oc = __shfl_xor(c, j, G);
+                const bool up = (lane & k) == 0;
+                const bool lower = (lane & j) == 0;
+                if ((lower == up) ? (oc < c) : (oc > c)) c = oc;
+            }
+        }
+        const uint32_t pc = __shfl_up(c, 1, G);
+        const int wl = threadIdx.x & 63;
+        const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << (wl - lane));
+        const uint64_t heads = __ballot(lane < n && (lane == 0 || pc != c));
+        if (lane == 0) u[a] = __popcll(heads & gmask);
+    } else {
+        SmallRow<G, B, VT> r;
+        if (!small_row_merge<G, B, VT>(p, a, lane, r)) return;   // long row: other kernel
+        if (r.head) {
+            int64_t rs[3];
+            for (int dr = 0; dr < dpn; ++dr) rs[dr] = (int64_t)row_ptr[a * dpn + dr];
+            small_row_store<B>(r.col, r.rank, r.v, rs, col_ind, val, dpn);
+        }
+    }
+}
+
+// Count, scan and write of small rows in ONE pass (2D meshes whose rows all
+// fit G lanes).  Blocks take tiles of FUSED_ITER * 256/G rows in the order
+// they start (ticket from an atomic counter), merge their rows as the write
+// pass does (results kept in registers), and find the rows' CSR offsets with
+// a decoupled look-back over the tiles started before them: each tile
+// publishes its aggregate, then its inclusive prefix once known; one wave
+// reads 64 predecessor states at a time, adds the aggregates back to the
+// nearest published prefix, and moves the window back by 64 while there is
+// none.  Every tile it waits on has started, so the waits end; they are
+// bounded anyway (spin limit -> error flag; the host then runs the separate
+// passes).  Saves the count pass: the keys are read and sorted once.
+// The tile states are relaxed atomics: each state word carries its own
+// payload, so no ordering of other memory is needed, and acquire/release at
+// device scope would write back or invalidate the XCD's caches per access.
+constexpr int64_t TILE_AGG = int64_t(1) << 62, TILE_PREFIX = int64_t(2) << 62;
+constexpr int64_t TILE_VALUE = TILE_AGG - 1;
+constexpr int FUSED_ITER = 4;
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t x) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+    return x;
+}
+
+template <int G, typename RP>
+__global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, int64_t n_tiles, RP* row_ptr,
+                                                    int32_t* col_ind, double* val) {
+    constexpr int RPI = 256 / G;                 // rows per iteration
+    constexpr int RPT = RPI * FUSED_ITER;        // rows per tile (<= 64)
+    constexpr int B = 4, D = 2;
+    static_assert(RPT <= 64, "one wave scans a tile's rows");
+    __shared__ int32_t s_tile;
+    __shared__ int64_t s_u[RPT];
+    __shared__ int64_t s_base;
+    if (threadIdx.x == 0) s_tile = atomicAdd(&p.misc[MISC_TICKET], 1);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= n_tiles) return;   // never: the grid has exactly n_tiles blocks
+    const int rib = threadIdx.x / G, lane = threadIdx.x % G;
+    SmallRow<G, B, double> r[FUSED_ITER];
+#pragma unroll
+    for (int it = 0; it < FUSED_ITER; ++it) {
+        const int64_t a = tile * RPT + it * RPI + rib;
+        r[it].u = 0;
+        r[it].head = false;
+        if (a < p.n_nodes && !small_row_merge<G, B, double>(p, a, lane, r[it])) {
+            // the host checked that no row is longer than G; never taken
+            if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], G + 1);
+            r[it].u = 0;
+            r[it].head = false;
+        }
+        if (lane == 0) s_u[it * RPI + rib] = a < p.n_nodes ? r[it].u : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int wl = threadIdx.x;
+        // Exclusive scan of the tile's row counts.
+        const int64_t x = wl < RPT ? s_u[wl] : 0;
+        int64_t incl = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t o = __shfl_up(incl, d, 64);
+            if (wl >= d) incl += o;
+        }
+        const int64_t agg = __shfl(incl, 63, 64);
+        if (wl < RPT) s_u[wl] = incl - x;
+        if (wl == 0)
+            __hip_atomic_store(&tstat[tile], (tile == 0 ? TILE_PREFIX : TILE_AGG) | agg, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        int64_t excl = 0;
+        for (int64_t j = tile - 1; j >= 0; j -= 64) {
+            const int64_t idx = j - wl;
+            int64_t st = idx >= 0 ? __hip_atomic_load(&tstat[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : TILE_PREFIX;
+            for (int32_t it = 0;; ++it) {
+                const bool pending = (st & ~TILE_VALUE) == 0;
+                if (!__any(pending)) break;
+                if (it >= ALLS_SPIN_MAX) {
+                    if (wl == 0) st_agent(&p.misc[MISC_ERR + ERR_LOOKBACK], 1);
+                    st = TILE_PREFIX;   // give up: the host discards this result
+                    continue;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                if (pending) st = __hip_atomic_load(&tstat[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const uint64_t pm = __ballot((st & TILE_PREFIX) != 0);
+            if (pm) {   // nearest predecessor with a prefix: stop there
+                const int first = __ffsll((unsigned long long)pm) - 1;
+                excl += wave_sum64(wl <= first ? (st & TILE_VALUE) : 0);
+                break;
+            }
+            excl += wave_sum64(st & TILE_VALUE);
+        }
+        if (wl == 0) {
+            if (tile > 0)
+                __hip_atomic_store(&tstat[tile], TILE_PREFIX | (excl + agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            s_base = excl;
+            if (tile == n_tiles - 1) {   // the last tile knows the total
+                const int64_t total = excl + agg;
+                row_ptr[p.n_nodes * D] = (RP)(total * D * D);
+                st_agent(&p.misc[MISC_TOTAL_LO], (int32_t)(total & 0x7fffffff));
+                st_agent(&p.misc[MISC_TOTAL_HI], (int32_t)(total >> 31));
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < FUSED_ITER; ++it) {
+        const int64_t a = tile * RPT + it * RPI + rib;
+        if (a >= p.n_nodes) continue;
+        const int64_t off = s_base + s_u[it * RPI + rib];
+        int64_t rs[D];
+#pragma unroll
+        for (int dr = 0; dr < D; ++dr) rs[dr] = D * (off * D + (int64_t)dr * r[it].u);
+        if (lane == 0) {
+#pragma unroll
+            for (int dr = 0; dr < D; ++dr) row_ptr[a * D + dr] = (RP)rs[dr];
+        }
+        if (r[it].head) small_row_store<B>(r[it].col, r[it].rank, r[it].v, rs, col_ind, val, D);
     }
 }
 
@@ -840,6 +987,28 @@ namespace alls {
 int32_t long_row_limit() { return LONG_CAP_MAX; }
 
 int64_t scan_tile_count(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+int64_t fused_tile_count(int64_t n_nodes, int G) {
+    const int64_t rpt = (int64_t)(256 / G) * FUSED_ITER;
+    return (n_nodes + rpt - 1) / rpt;
+}
+
+void launch_rows_fused(const ListPool& p, int G, int64_t* tstat, void* row_ptr, int i64, int32_t* col_ind,
+                       double* val, hipStream_t s) {
+    const int64_t nt = fused_tile_count(p.n_nodes, G);
+    if (nt <= 0) return;
+#define ALLS_FU(GG, RP)                                                                                    \
+    hipLaunchKernelGGL((k_rows_fused<GG, RP>), dim3((unsigned)nt), dim3(256), 0, s, p, tstat, nt,            \
+                       static_cast<RP*>(row_ptr), col_ind, val)
+    if (G == 16) {
+        if (i64) ALLS_FU(16, int64_t);
+        else ALLS_FU(16, int32_t);
+    } else {
+        if (i64) ALLS_FU(32, int64_t);
+        else ALLS_FU(32, int32_t);
+    }
+#undef ALLS_FU
+}
 
 void launch_count(const ListPool& p, int G, int f32vals, int32_t* u, int32_t* big_list, hipStream_t s) {
     if (p.B == 4)
