@@ -128,6 +128,16 @@ ALLS_DEV int32_t list_reserve(const ListPool& p, int32_t a, int32_t n) {
     return pos;
 }
 
+// list_reserve in two halves, for a thread that reserves in several rows:
+// the reservation atomics go out back to back (no branch on a result
+// between them), and the long-row note is taken once all have returned.
+ALLS_DEV int32_t list_reserve_issue(const ListPool& p, int32_t a, int32_t n) {
+    return add_agent(&p.hdr[(int64_t)a * ALLS_DIR], n);
+}
+ALLS_DEV void list_note_long(const ListPool& p, int32_t pos, int32_t n) {
+    if (pos + n > p.long_row) max_agent(&p.misc[MISC_MAX_ROW], pos + n);
+}
+
 // The reservation [pos, pos+n) owns every chunk k >= 1 whose first slot
 // k*C lies inside it: take that chunk from the pool and publish it.
 ALLS_DEV void list_own_chunks(const ListPool& p, int32_t a, int32_t pos, int32_t n) {

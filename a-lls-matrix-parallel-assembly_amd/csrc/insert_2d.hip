@@ -17,6 +17,7 @@
 #include "elem2d.h"
 
 using alls_dev::Elem2D;
+using alls_dev::RowMoments;
 
 namespace {
 
@@ -57,11 +58,18 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
         Elem2D el;
         int nn = 0;
         int32_t pos[4] = {0, 0, 0, 0};
-        if (e < m.n_elems && el.load<MODE>(m, e, p.misc)) {
+        if (e < m.n_elems && el.load_geom<MODE>(m, e, p.misc)) {
             nn = el.nn;
+            // The reservations need only the node ids: issue them before the
+            // integration so that their round trips overlap its arithmetic.
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (i < nn) pos[i] = list_reserve(p, el.nd[i], nn);
+                if (i < nn) pos[i] = list_reserve_issue(p, el.nd[i], nn);
+            el.moments();
+            el.settle();
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (i < nn) list_note_long(p, pos[i], nn);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
                 if (i < nn) list_own_chunks(p, el.nd[i], pos[i], nn);
@@ -150,8 +158,14 @@ __global__ __launch_bounds__(256) void k_scatter_2d(DevMesh m, Scatter sc, int32
 // Value-only re-assembly of a cached 2D pattern without atomics: one thread
 // owns node row a, walks the elements incident to a in ascending element id
 // (the serial assembly's order, Acc:336), recomputes each element's moments
-// and adds its blocks K_ij (i = local index of a) straight into the CSR rows
-// a*2 and a*2+1, which no other thread touches.
+// and adds its blocks K_ij (i = local index of a) into the row's blocks.
+// A row of at most GATHER_SLOTS block columns (every row of a mesh built
+// from quadrilateral cells, split or not) accumulates in registers: block
+// slot s is updated by a predicated add, and the row is stored once at the
+// end.  Longer rows add into the CSR rows a*2 and a*2+1 in memory, which no
+// other thread touches.  Both forms sum 0 + v1 + v2 + ... in the same order.
+constexpr int GATHER_SLOTS = 9;
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const int64_t* inc_off,
                                                    const int32_t* inc_list, int32_t* misc) {
@@ -159,18 +173,65 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
     if (a >= m.n_nodes) return;
     const int64_t r0 = rp_at(sc.row_ptr, sc.i64, 2 * a);
     const int64_t r1 = rp_at(sc.row_ptr, sc.i64, 2 * a + 1);
+    const int64_t rowlen = r1 - r0;
+    const int64_t k0 = inc_off[a], k1 = inc_off[a + 1];
+    if (rowlen <= 2 * GATHER_SLOTS) {
+        double acc[GATHER_SLOTS][4];
+#pragma unroll
+        for (int q = 0; q < GATHER_SLOTS; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0;
+        for (int64_t k = k0; k < k1; ++k) {
+            const int64_t e = inc_list[k];
+            Elem2D el;
+            if (!el.load_geom<MODE>(m, e, misc)) continue;
+            el.material();
+            // Only row node i's moments are formed (i is a run-time index);
+            // the accumulators are indexed by constants only and stay in
+            // registers.
+#pragma unroll 1
+            for (int i = 0; i < el.nn; ++i) {
+                if (el.nd[i] != a) continue;
+                RowMoments rm;
+                el.row_moments(i, rm);
+                const int4 mp = *reinterpret_cast<const int4*>(sc.map + e * 16 + i * 4);
+                const int32_t sl[4] = {mp.x, mp.y, mp.z, mp.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (j >= el.nn) continue;
+                    double kb[4];
+                    el.row_block(rm, j, kb);
+#pragma unroll
+                    for (int q = 0; q < GATHER_SLOTS; ++q) {
+                        if (sl[j] == q) {
+                            acc[q][0] += kb[0];
+                            acc[q][1] += kb[1];
+                            acc[q][2] += kb[2];
+                            acc[q][3] += kb[3];
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < GATHER_SLOTS; ++q) {
+            if (2 * q < rowlen) {
+                *reinterpret_cast<double2*>(sc.val + r0 + 2 * q) = make_double2(acc[q][0], acc[q][1]);
+                *reinterpret_cast<double2*>(sc.val + r1 + 2 * q) = make_double2(acc[q][2], acc[q][3]);
+            }
+        }
+        return;
+    }
     const int64_t r2 = rp_at(sc.row_ptr, sc.i64, 2 * a + 2);
     for (int64_t q = r0; q < r2; ++q) sc.val[q] = 0.0;
-    const int64_t rowlen = r1 - r0;
-    for (int64_t k = inc_off[a]; k < inc_off[a + 1]; ++k) {
+    for (int64_t k = k0; k < k1; ++k) {
         const int64_t e = inc_list[k];
         Elem2D el;
-        if (!el.load<MODE>(m, e, misc)) continue;
-        // Static loops with a predicate on the local index of a: the moment
-        // arrays are indexed by constants only and stay in registers.
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (i >= el.nn || el.nd[i] != a) continue;
+        if (!el.load_geom<MODE>(m, e, misc)) continue;
+        el.material();
+#pragma unroll 1
+        for (int i = 0; i < el.nn; ++i) {
+            if (el.nd[i] != a) continue;
+            RowMoments rm;
+            el.row_moments(i, rm);
             const int32_t* mp = sc.map + e * 16 + i * 4;
             // All loads of the element's blocks first, then all stores: the
             // element's columns are distinct, and issuing the loads together
@@ -189,7 +250,7 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
             for (int j = 0; j < 4; ++j) {
                 if (j >= el.nn) continue;
                 double kb[4];
-                el.block(i, j, kb);
+                el.row_block(rm, j, kb);
                 *reinterpret_cast<double2*>(v[j]) = make_double2(old[j][0] + kb[0], old[j][1] + kb[1]);
                 *reinterpret_cast<double2*>(v[j] + rowlen) = make_double2(old[j][2] + kb[2], old[j][3] + kb[3]);
             }

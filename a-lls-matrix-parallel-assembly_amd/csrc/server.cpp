@@ -150,7 +150,7 @@ struct alls_server {
     std::unordered_map<uint64_t, int> seen;   // admitted requests per mesh key
     // Per-element device cost (ns) of the plain, build and hit paths, per
     // dimension (0: 2D, 1: Hex20), as running averages of served requests.
-    double cost[2][3] = {{0.7, 1.2, 0.55}, {35.0, 120.0, 30.0}};
+    double cost[2][3] = {{0.7, 1.2, 0.35}, {35.0, 120.0, 30.0}};
 
     enum { PATH_PLAIN = 0, PATH_BUILD = 1, PATH_HIT = 2 };
     // Whether a request of a key seen `prior` times should build its pattern.
