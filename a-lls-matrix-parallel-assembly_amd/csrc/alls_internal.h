@@ -3,12 +3,23 @@
 // Data layout in HBM (one assembly):
 //   hdr   int32 [n_nodes][8]   hdr[a][0] = entries appended to node-row a,
 //                              hdr[a][k] = 1 + index of overflow chunk k (0 = none), k = 1..7
-//   keys  int2  [chunks][C]    (column node b, element id e) of every entry
 //   vals  VT    [chunks][C][B] B = dpn*dpn block of the entry (VT = double, or
 //                              float for FP32-integrated Hex20)
+//   Q4/T3: geid  int32 [chunks][C/ng]  element id of each group of ng slots
+//   mixed 2D and Hex20:
+//          keys  int2  [chunks][C]     (column node b, element id e) of every entry
 // Chunk a (a < n_nodes) is node-row a's home chunk; overflow chunks are taken
 // from a bump counter above n_nodes.  Rows are node rows: one entry carries the
 // dpn x dpn block that an element adds to DOF rows a*dpn.. and DOF columns b*dpn..
+//
+// In a Q4 or T3 mesh an element reserves ng = nn slots (4 or 3) in each of
+// its node rows, and entry q of the group is its block with column node
+// conn[e][q]: the group's slots need no key of their own, only the group's
+// element id.  Chunks hold whole groups (C is a multiple of ng).  Keys for
+// the rows that the wave-per-row kernels take are written out by the count
+// pass (keys[] is kept for that).  A mixed mesh's groups would differ in size
+// (padding its triangles to 4 slots measured 25% slower), so its entries keep
+// per-entry keys.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -59,12 +70,19 @@ struct ListPool {
     int2* keys;
     void* vals;
     int32_t* misc;
+    int32_t* geid;             // 2D: element id of each slot group
+    const int32_t* conn;       // 2D: the mesh's connectivity (column nodes of a group)
     int64_t n_nodes;           // home chunks
     int64_t cap_chunks;        // home + overflow capacity
+    int64_t n_elems;           // 2D: bound of a group's element id
     int32_t C;                 // entries per chunk (<= ALLS_MAX_CHUNK)
     int32_t B;                 // values per entry
     uint32_t cmagic;           // chunk_magic(C): s / C as a multiply-high
     int32_t long_row;          // reservations ending past this many entries record MISC_MAX_ROW
+    int32_t ng;                // Q4 / T3: slots per group (4 or 3); 0: per-entry keys
+    int32_t gpc;               // 2D: groups per chunk (C / ng)
+    int32_t npe;               // 2D: conn stride
+    int32_t _pad;
 };
 
 #define ALLS_MAX_CHUNK 16384
@@ -178,6 +196,46 @@ ALLS_DEV int64_t list_chunk(const ListPool& p, int32_t a, int32_t s) {
     }
     if (v < 0) return -1;
     return p.n_nodes + (int64_t)(v - 1);
+}
+
+// Keyed lists (mixed 2D meshes): the keys (column node, element) of one
+// element's nn entries in row a, slots [pos, pos+nn), stored by the thread
+// that reserved them, straight from its registers.
+ALLS_DEV void list_store_keys(const ListPool& p, int32_t a, int32_t pos, int nn, const int32_t* nd, int32_t e) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= nn) continue;
+        const int32_t s = pos + j;
+        const int64_t ch = list_chunk(p, a, s);
+        if (ch >= 0) p.keys[ch * p.C + (s - chunk_of(p, s) * p.C)] = make_int2(nd[j], e);
+    }
+}
+
+// r / ng for a slot r inside a chunk (ng = 3 or 4).
+ALLS_DEV int32_t group_of(const ListPool& p, int32_t r) {
+    return p.ng == 4 ? (r >> 2) : (int32_t)(__umulhi((uint32_t)r, 0xAAAAAAABu) >> 1);
+}
+
+// 2D: records element e as the owner of the group at slot pos of row a (the
+// group lies inside one chunk).  One 4-byte store per element row.
+ALLS_DEV void list_store_group(const ListPool& p, int32_t a, int32_t pos, int32_t e) {
+    const int64_t ch = list_chunk(p, a, pos);
+    if (ch < 0) return;
+    const int32_t r = pos - chunk_of(p, pos) * p.C;
+    p.geid[ch * p.gpc + group_of(p, r)] = e;
+}
+
+// Q4 / T3: the key of slot r (any slot of the row with the same r % ng)
+// given its group's element e: the column node is conn[e][r % ng].  An
+// element id out of range (a group left unwritten after a failed chunk wait,
+// which the host then retries) reads as column a with *pad set (its value
+// counts as zero), so no access leaves the arrays.
+ALLS_DEV int2 group_key(const ListPool& p, int64_t a, int32_t r, int32_t e, bool* pad) {
+    const int32_t q = r - group_of(p, r) * p.ng;
+    const bool bad = (uint64_t)(uint32_t)e >= (uint64_t)p.n_elems;
+    const int32_t b = bad ? -1 : p.conn[(int64_t)e * p.npe + q];
+    *pad = b < 0;
+    return make_int2(b < 0 ? (int32_t)a : b, bad ? 0 : e);
 }
 
 #endif  // __HIPCC__

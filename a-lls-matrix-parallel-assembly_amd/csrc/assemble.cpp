@@ -356,7 +356,7 @@ hipError_t Workspace::upload(void* const* dst, const void* const* src, const siz
 
 size_t Workspace::device_bytes() const {
     size_t b = 0;
-    for (const DevBuf* x : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
+    for (const DevBuf* x : {&hdr, &keys, &geid, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b += x->bytes;
     return b;
 }
@@ -364,14 +364,14 @@ size_t Workspace::device_bytes() const {
 void Workspace::release_device() {
     g_front.reset();
     g_back.reset();
-    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
+    for (DevBuf* b : {&hdr, &keys, &geid, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b->release();
 }
 
 void Workspace::destroy() {
     g_front.reset();
     g_back.reset();
-    for (DevBuf* b : {&hdr, &keys, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
+    for (DevBuf* b : {&hdr, &keys, &geid, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b->release();
     if (host_misc) (void)hipHostFree(host_misc);
     host_misc = nullptr;
@@ -615,7 +615,12 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     const int64_t ovf = pl.full_pool ? bound : std::min(bound, bound / 4 + n_nodes / 4 + 1024);
     pl.cap_chunks = n_nodes + ovf;
     HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
+    // Q4 / T3 lists carry one element id per group of ng slots; the
+    // per-entry keys are written only for the rows the wave-per-row kernels
+    // take.  Mixed and Hex20 lists key every entry.
+    const int ng = mi.kind == 0 ? 4 : mi.kind == 1 ? 3 : 0;
     HIP_TRY(ws.keys.ensure((size_t)pl.cap_chunks * pl.C * sizeof(int2)), "alloc list keys");
+    if (ng) HIP_TRY(ws.geid.ensure((size_t)pl.cap_chunks * (pl.C / ng) * sizeof(int32_t)), "alloc list groups");
     HIP_TRY(ws.vals.ensure((size_t)pl.cap_chunks * pl.C * B * vsz), "alloc list values");
     HIP_TRY(ws.misc.ensure(MISC_SIZE * sizeof(int32_t)), "alloc misc");
     HIP_TRY(ws.u.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc row counts");
@@ -631,6 +636,12 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     pl.lp.B = B;
     pl.lp.cmagic = chunk_magic(pl.C);
     pl.lp.long_row = pl.G > 0 ? pl.G : INT32_MAX;
+    pl.lp.ng = ng;
+    pl.lp.gpc = ng ? pl.C / ng : 0;
+    pl.lp.geid = ng ? ws.geid.as<int32_t>() : nullptr;
+    pl.lp.conn = dm.conn;
+    pl.lp.n_elems = dm.n_elems;
+    pl.lp.npe = dm.npe;
     return ALLS_OK;
 }
 
@@ -801,7 +812,8 @@ uint64_t front_key(const Workspace& ws, const DevMesh& dm, const MeshInfo& mi, c
                    const Pipeline& pl) {
     uint64_t h = 0x5eed;
     for (const void* p : {(const void*)dm.coords, (const void*)dm.conn, (const void*)dm.types,
-                          (const void*)dm.E_elem, (const void*)pl.lp.hdr, (const void*)pl.lp.keys, (const void*)pl.lp.vals,
+                          (const void*)dm.E_elem, (const void*)pl.lp.hdr, (const void*)pl.lp.keys,
+                          (const void*)pl.lp.geid, (const void*)pl.lp.vals,
                           (const void*)pl.lp.misc, (const void*)ws.u.p, (const void*)ws.big.p, (const void*)ws.tiles.p,
                           (const void*)ws.host_misc})
         h = mixp(h, p);
@@ -815,12 +827,14 @@ uint64_t front_key(const Workspace& ws, const DevMesh& dm, const MeshInfo& mi, c
 uint64_t back_key(const Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const Pipeline& pl,
                   const alls_csr* out) {
     uint64_t h = 0xbac;
-    for (const void* p : {(const void*)pl.lp.hdr, (const void*)pl.lp.keys, (const void*)pl.lp.vals, (const void*)pl.lp.misc,
+    for (const void* p : {(const void*)pl.lp.hdr, (const void*)pl.lp.keys, (const void*)pl.lp.geid,
+                          (const void*)pl.lp.conn, (const void*)pl.lp.vals, (const void*)pl.lp.misc,
                           (const void*)ws.u.p, (const void*)ws.big.p, (const void*)ws.tiles.p, (const void*)out->row_ptr,
                           (const void*)out->col_ind, (const void*)out->val})
         h = mixp(h, p);
     for (int64_t v : {dm.n_nodes, (int64_t)mi.dpn, (int64_t)mi.f32, (int64_t)pl.C, pl.cap_chunks, pl.ntiles,
-                      (int64_t)pl.G, (int64_t)pl.max_big, (int64_t)out->row_ptr_is_i64})
+                      (int64_t)pl.G, (int64_t)pl.max_big, (int64_t)out->row_ptr_is_i64, (int64_t)dm.npe,
+                      (int64_t)mi.kind})
         h = mix(h, (uint64_t)v);
     return h;
 }
@@ -868,6 +882,8 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     Pipeline pl;
     pl.C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
     pl.C = std::min(std::max(pl.C, 4), ALLS_MAX_CHUNK);
+    if (mi.kind == 1) pl.C = std::max(3, pl.C / 3 * 3);   // whole slot groups per chunk
+    else if (mi.kind == 0) pl.C = std::max(4, pl.C / 4 * 4);
     pl.G = small_group(mi.kind);
     pl.ntiles = scan_tile_count(n_nodes);
     // Graph replays do not record the per-stage events; the whole front and
@@ -953,7 +969,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
         st->list_entries = mi.entries_bound;
         st->chunk_slots = pl.C;
         st->retries = pl.retries;
-        st->scratch_bytes = (int64_t)(ws.hdr.bytes + ws.keys.bytes + ws.vals.bytes);
+        st->scratch_bytes = (int64_t)(ws.hdr.bytes + ws.keys.bytes + ws.geid.bytes + ws.vals.bytes);
         const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
         st->big_rows = f[MISC_BIG_COUNT];
         st->pool_chunks_used = n_nodes + f[MISC_OVF_TOP];

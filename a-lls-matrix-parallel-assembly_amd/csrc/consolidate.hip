@@ -40,6 +40,38 @@ __device__ __forceinline__ int64_t entry_index(const ListPool& p, int64_t a, con
     return ch * p.C + (s - k * p.C);
 }
 
+// Chunk and in-chunk slot of slot s of row a (as entry_index).
+__device__ __forceinline__ void entry_loc(const ListPool& p, int64_t a, const int32_t* h, int32_t s, int64_t* ch,
+                                          int32_t* r) {
+    const int32_t k = chunk_of(p, s);
+    const int32_t d = k == 0 ? 0 : h[k];
+    *ch = d > 0 ? p.n_nodes + (int64_t)(d - 1) : a;
+    *r = s - k * p.C;
+}
+
+// 2D lists: writes the (column, element) keys of a row that the wave-per-row
+// kernels will take (more than G entries), from the group element ids, and
+// zeroes the value of each unused triangle slot (read as column a).  Lanes
+// 0..G-1 of the row's group take slots lane, lane+G, ...
+template <int B, typename VT>
+__device__ __forceinline__ void materialize_keys(const ListPool& p, int64_t a, int lane, int G, int32_t n) {
+    const int32_t* h = p.hdr + a * ALLS_DIR;
+    for (int32_t sl = lane; sl < n; sl += G) {
+        int64_t ch;
+        int32_t r;
+        entry_loc(p, a, h, sl, &ch, &r);
+        bool pad;
+        const int2 key = group_key(p, a, r, p.geid[ch * p.gpc + group_of(p, r)], &pad);
+        const int64_t idx = ch * p.C + r;
+        p.keys[idx] = key;
+        if (pad) {
+            VT* v = static_cast<VT*>(p.vals) + idx * B;
+#pragma unroll
+            for (int b = 0; b < B; ++b) v[b] = VT(0);
+        }
+    }
+}
+
 // entry_index with the row's directory already in registers (wave-uniform,
 // loaded once per row): no memory access per entry.
 struct RowDir {
@@ -88,7 +120,8 @@ template <int B, typename VT, bool VALS>
 __device__ __forceinline__ void small_row_load(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L) {
     L.kh = make_int2(0, 0);
     if (lane < p.C) {
-        L.kh = p.keys[a * p.C + lane];
+        if (p.ng) L.kh.y = p.geid[a * p.gpc + group_of(p, lane)];   // 2D: the group's element
+        else L.kh = p.keys[a * p.C + lane];
         if constexpr (VALS) {
             const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + lane) * B;
 #pragma unroll
@@ -98,19 +131,44 @@ __device__ __forceinline__ void small_row_load(const ListPool& p, int64_t a, int
     L.n = p.hdr[a * ALLS_DIR];
 }
 
+// Second step after the loads: the row's entry count (clamped to the
+// directory), the entries of overflow chunks, and for 2D lists each slot's
+// column from its group's element.  Rows are resolved a few at a time before
+// any is merged, so that these dependent loads overlap too.  A row longer
+// than G is left alone (the wave-per-row kernels take it).
 template <int G, int B, typename VT>
-__device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L,
-                                                SmallRow<G, B, VT>& r) {
+__device__ __forceinline__ void small_row_keys(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L) {
     const int32_t n = min(L.n, ALLS_DIR * p.C);
-    if (n > G) return false;
-    int2 kh = L.kh;
+    L.n = n;
+    if (n > G) return;
     if (lane >= p.C && lane < n) {   // overflow chunk (rare for rows this short)
-        const int64_t ei = entry_index(p, a, p.hdr + a * ALLS_DIR, lane);
-        kh = p.keys[ei];
+        int64_t ch;
+        int32_t r;
+        entry_loc(p, a, p.hdr + a * ALLS_DIR, lane, &ch, &r);
+        const int64_t ei = ch * p.C + r;
+        if (p.ng) L.kh.y = p.geid[ch * p.gpc + group_of(p, r)];
+        else L.kh = p.keys[ei];
         const VT* src = static_cast<const VT*>(p.vals) + ei * B;
 #pragma unroll
         for (int b = 0; b < B; ++b) L.vh[b] = src[b];
     }
+    if (p.ng && lane < n) {   // 2D: column from the group's element
+        bool pad;
+        L.kh = group_key(p, a, lane, L.kh.y, &pad);
+        if (pad) {
+#pragma unroll
+            for (int b = 0; b < B; ++b) L.vh[b] = VT(0);
+        }
+    }
+}
+
+// After small_row_keys.  Returns false for a row longer than G.
+template <int G, int B, typename VT>
+__device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L,
+                                                SmallRow<G, B, VT>& r) {
+    const int32_t n = L.n;
+    if (n > G) return false;
+    const int2 kh = L.kh;
     const int wl = threadIdx.x & 63;
     const int gbase = wl - lane;
     const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
@@ -200,6 +258,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     if (wbase >= p.n_nodes) return;
     if constexpr (!WRITE) {
         // Counting needs only the columns: a bitonic sort of 32-bit keys.
+        // kc: the entry's column (keyed lists) or its group's element (2D).
         int32_t kc[R], hn[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -207,20 +266,38 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             kc[r] = 0;
             hn[r] = 0;
             if (a < p.n_nodes) {
-                if (lane < p.C) kc[r] = p.keys[a * p.C + lane].x;   // with the header, not after it
-                hn[r] = p.hdr[a * ALLS_DIR];
+                if (lane < p.C)   // with the header, not after it
+                    kc[r] = p.ng ? p.geid[a * p.gpc + group_of(p, lane)] : p.keys[a * p.C + lane].x;
+                hn[r] = min(p.hdr[a * ALLS_DIR], ALLS_DIR * p.C);
+            }
+        }
+        // Overflow entries and 2D columns of every round, before any sort.
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t a = wbase + r * GPW;
+            const int32_t n = hn[r];
+            if (a >= p.n_nodes || n > G) continue;
+            if (lane >= p.C && lane < n) {
+                int64_t ch;
+                int32_t rr;
+                entry_loc(p, a, p.hdr + a * ALLS_DIR, lane, &ch, &rr);
+                kc[r] = p.ng ? p.geid[ch * p.gpc + group_of(p, rr)] : p.keys[ch * p.C + rr].x;
+            }
+            if (p.ng && lane < n) {
+                bool pad;
+                kc[r] = group_key(p, a, lane, kc[r], &pad).x;
             }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int64_t a = wbase + r * GPW;
             if (a >= p.n_nodes) break;
-            const int32_t n = min(hn[r], ALLS_DIR * p.C);
+            const int32_t n = hn[r];
             if (n > G) {
                 if (lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
+                if (p.ng) materialize_keys<B, VT>(p, a, lane, G, n);
                 continue;
             }
-            if (lane >= p.C && lane < n) kc[r] = p.keys[entry_index(p, a, p.hdr + a * ALLS_DIR, lane)].x;
             uint32_t c = lane < n ? (uint32_t)kc[r] : 0xffffffffu;
 #pragma unroll
             for (int k = 2; k <= G; k <<= 1) {
@@ -243,6 +320,9 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (wbase + r * GPW < p.n_nodes) small_row_load<B, VT, true>(p, wbase + r * GPW, lane, L[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (wbase + r * GPW < p.n_nodes) small_row_keys<G, B, VT>(p, wbase + r * GPW, lane, L[r]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int64_t a = wbase + r * GPW;
@@ -303,6 +383,11 @@ __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, 
     for (int it = 0; it < FUSED_ITER; ++it) {
         const int64_t a = tile * RPT + it * RPI + rib;
         if (a < p.n_nodes) small_row_load<B, double, true>(p, a, lane, L[it]);
+    }
+#pragma unroll
+    for (int it = 0; it < FUSED_ITER; ++it) {
+        const int64_t a = tile * RPT + it * RPI + rib;
+        if (a < p.n_nodes) small_row_keys<G, B, double>(p, a, lane, L[it]);
     }
 #pragma unroll
     for (int it = 0; it < FUSED_ITER; ++it) {

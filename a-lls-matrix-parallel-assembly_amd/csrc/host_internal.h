@@ -66,7 +66,7 @@ struct GraphSlot {
 // Per-stream scratch: list headers, pool, consolidation arrays and a device
 // copy of host meshes.  A context owns one; the server owns one per slot.
 struct Workspace {
-    DevBuf hdr, keys, vals, misc, u, big, tiles;
+    DevBuf hdr, keys, geid, vals, misc, u, big, tiles;
     DevBuf coords, conn, types, E;
     DevBuf bdesc;                   // batch: source-mesh table, boundary indices and offsets
     DevBuf bsplit;                  // batch split: per-request output table (server)

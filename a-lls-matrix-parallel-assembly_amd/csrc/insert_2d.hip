@@ -23,11 +23,10 @@ namespace {
 
 // One wave's LDS stage of one row slot i: every lane's (up to) 4 entries of
 // its element's row node nd[i].
-// Rows are padded (9 x 16 B, 5 x 8 B) so that the lane-strided staging
-// writes spread over the LDS banks.
+// Rows are padded (9 x 16 B) so that the lane-strided staging writes spread
+// over the LDS banks.
 struct RowStage {
     double2 v[64][9];      // entry j as (K00, K01), (K10, K11); [8] is padding
-    int2 key[64][5];       // [4] is padding
     int32_t row[64], pos[64], nn[64];
 };
 
@@ -37,12 +36,17 @@ struct RowStage {
 // node rows (phase 1), then publishes the overflow chunks its reservations
 // own (phase 2); a chunk is always published by the wave that reserved it,
 // before that wave waits on anything, so no wait can hang on a wave that is
-// not running.  The entries are then written row slot by row slot through
+// not running.  The values are then written row slot by row slot through
 // LDS: 8 lanes store one element's row group (4 entries x 32 B, one 128-B
-// segment) and 4 lanes its keys, instead of every lane scattering its own
-// 16-B pieces over 64 rows.  Scattered stores kept the address unit busy
-// (rocprof TA_TA_BUSY ~80% of the kernel); grouped stores touch about a sixth
-// of the cache lines per instruction.
+// segment), instead of every lane scattering its own 16-B pieces over 64
+// rows.  Scattered stores kept the address unit busy (rocprof TA_TA_BUSY ~80%
+// of the kernel); grouped stores touch about a sixth of the cache lines per
+// instruction.  For Q4 and T3 meshes the key is the group's element id (4 B
+// per element row, stored straight from the element's lane): the column of
+// entry j is the element's node j, so the entries carry no key of their own.
+// Per-entry keys cost 0.27 ms of 0.85 at Q4 2000x2000 (4M extra cache lines
+// touched by partial writes); the element ids touch 0.5M.  Mixed meshes
+// (groups of 3 and 4) keep per-entry keys, stored by the element's lane.
 //
 // Grid-stride over blocks of elements (persistent grid, or one thread per
 // element): `base` is uniform over the block, so every wave takes the same
@@ -89,9 +93,10 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                         el.block(i, j, k);
                         st.v[lane][2 * j] = make_double2(k[0], k[1]);
                         st.v[lane][2 * j + 1] = make_double2(k[2], k[3]);
-                        st.key[lane][j] = make_int2(el.nd[j], (int32_t)e);
                     }
                 }
+                if (MODE == 2) list_store_keys(p, el.nd[i], pos[i], nn, el.nd, (int32_t)e);
+                else list_store_group(p, el.nd[i], pos[i], (int32_t)e);
             }
             __syncthreads();
 #pragma unroll
@@ -105,16 +110,6 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                         const int64_t idx = ch * p.C + (s - chunk_of(p, s) * p.C);
                         reinterpret_cast<double2*>(p.vals)[idx * 2 + (w & 1)] = st.v[src][w];
                     }
-                }
-            }
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {   // keys: lane -> (element src, entry j)
-                const int piece = t * 64 + lane;
-                const int src = piece >> 2, j = piece & 3;
-                if (j < st.nn[src]) {
-                    const int32_t s = st.pos[src] + j;
-                    const int64_t ch = list_chunk(p, st.row[src], s);
-                    if (ch >= 0) p.keys[ch * p.C + (s - chunk_of(p, s) * p.C)] = st.key[src][j];
                 }
             }
             __syncthreads();

@@ -79,7 +79,7 @@ int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pat
     const int64_t chunks = m->n_nodes + std::min(bound, bound / 4 + m->n_nodes / 4 + 1024);
     const int64_t n_rows = m->n_nodes * mi.dpn;
     const int64_t nnz_bound = ent * B;
-    int64_t scratch = chunks * C * (8 + B * vsz) + m->n_nodes * (32 + 8) + 1024;
+    int64_t scratch = chunks * C * (8 + 2 + B * vsz) + m->n_nodes * (32 + 8) + 1024;   // keys, group ids, values
     int64_t inputs = m->n_nodes * m->dim * 8 + m->n_elems * (m->nodes_per_elem * 4 + 9);
     int64_t out = (n_rows + 1) * 8 + nnz_bound * 12;
     int64_t pat = 0;
