@@ -30,6 +30,65 @@ __device__ __forceinline__ uint64_t pack_key(int2 kv) {
 }
 __device__ __forceinline__ int32_t key_col(uint64_t k) { return (int32_t)(k >> 32); }
 
+// Cross-lane moves inside 16-lane rows by DPP (a VALU operand modifier, no
+// LDS round trip), with ds_bpermute (__shfl_*) beyond them.  The sorts and
+// run sums below call these with constant distances once unrolled.
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+// Value of lane (lane ^ j) of the same G-lane group.
+__device__ __forceinline__ uint32_t xor_u32(uint32_t v, int j, int G) {
+    switch (j) {
+        case 1: return dpp_u32<0xB1>(v);                   // quad_perm [1,0,3,2]
+        case 2: return dpp_u32<0x4E>(v);                   // quad_perm [2,3,0,1]
+        case 4: return dpp_u32<0x1B>(dpp_u32<0x141>(v));   // row_half_mirror, then quad_perm [3,2,1,0]
+        case 8: return dpp_u32<0x128>(v);                  // row_ror:8
+        default: return (uint32_t)__shfl_xor((int)v, j, G);
+    }
+}
+__device__ __forceinline__ uint64_t xor_u64(uint64_t v, int j, int G) {
+    return ((uint64_t)xor_u32((uint32_t)(v >> 32), j, G) << 32) | xor_u32((uint32_t)v, j, G);
+}
+// Value of lane (lane + d) of the same group, d >= 1; garbage where lane + d
+// leaves the group (callers mask those lanes).  DPP row_shl for G == 16.
+template <int G>
+__device__ __forceinline__ uint32_t down_u32(uint32_t v, int d) {
+    if constexpr (G == 16) {
+        switch (d) {
+            case 1: return dpp_u32<0x101>(v);
+            case 2: return dpp_u32<0x102>(v);
+            case 3: return dpp_u32<0x103>(v);
+            case 4: return dpp_u32<0x104>(v);
+            case 5: return dpp_u32<0x105>(v);
+            case 6: return dpp_u32<0x106>(v);
+            case 7: return dpp_u32<0x107>(v);
+            case 8: return dpp_u32<0x108>(v);
+            case 9: return dpp_u32<0x109>(v);
+            case 10: return dpp_u32<0x10A>(v);
+            case 11: return dpp_u32<0x10B>(v);
+            case 12: return dpp_u32<0x10C>(v);
+            case 13: return dpp_u32<0x10D>(v);
+            case 14: return dpp_u32<0x10E>(v);
+            default: return dpp_u32<0x10F>(v);
+        }
+    }
+    const int lane = (int)(threadIdx.x & (G - 1));
+    return (uint32_t)__shfl((int)v, lane + d, G);
+}
+// Value of lane (lane - 1) of the same group (lane 0: garbage).
+template <int G>
+__device__ __forceinline__ uint32_t up1_u32(uint32_t v) {
+    if constexpr (G == 16) return dpp_u32<0x111>(v);   // row_shr:1
+    return (uint32_t)__shfl_up((int)v, 1, G);
+}
+template <int G>
+__device__ __forceinline__ double down_f64(double x, int d) {
+    const uint64_t b = __double_as_longlong(x);
+    const uint64_t r = ((uint64_t)down_u32<G>((uint32_t)(b >> 32), d) << 32) | down_u32<G>((uint32_t)b, d);
+    return __longlong_as_double(r);
+}
+
 __device__ __forceinline__ int64_t entry_index(const ListPool& p, int64_t a, const int32_t* h, int32_t s) {
     // Callers clamp s below ALLS_DIR * C.  A chunk that was never published
     // (only after a directory overflow, which the host then retries) maps to
@@ -179,8 +238,8 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint64_t ok = __shfl_xor(key, j, G);
-            const int32_t os = __shfl_xor(slot, j, G);
+            const uint64_t ok = xor_u64(key, j, G);
+            const int32_t os = (int32_t)xor_u32((uint32_t)slot, j, G);
             const bool up = (lane & k) == 0;
             const bool lower = (lane & j) == 0;
             const bool take = (lower == up) ? (ok < key) : (ok > key);
@@ -192,7 +251,7 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
     }
     r.col = key_col(key);
     const bool valid = lane < n;
-    const int32_t pcol = key_col(__shfl_up(key, 1, G));
+    const int32_t pcol = (int32_t)up1_u32<G>((uint32_t)(key >> 32));
     r.head = valid && (lane == 0 || pcol != r.col);
     const uint64_t gheads = __ballot(r.head) & gmask;
     r.u = __popcll(gheads);
@@ -205,14 +264,27 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
     }
     const uint64_t above = wl == 63 ? 0ull : (gheads & (~0ull << (wl + 1)));
     const int32_t run = r.head ? (above ? (__ffsll((unsigned long long)above) - 1 - gbase) : n) - lane : 0;
-    for (int d = 1; d < G; ++d) {
-        if (!__any(d < run)) break;
+    // Run sums: the head lane adds the run's values one by one.  Unrolled for
+    // 16-lane rows (DPP needs constant distances).
+    auto step = [&](int d) {
         double x[B];
 #pragma unroll
-        for (int b = 0; b < B; ++b) x[b] = __shfl(r.v[b], lane + d, G);
+        for (int b = 0; b < B; ++b) x[b] = down_f64<G>(r.v[b], d);
         if (d < run) {
 #pragma unroll
             for (int b = 0; b < B; ++b) r.v[b] += x[b];
+        }
+    };
+    if constexpr (G == 16) {
+#pragma unroll
+        for (int d = 1; d < G; ++d) {
+            if (!__any(d < run)) break;
+            step(d);
+        }
+    } else {
+        for (int d = 1; d < G; ++d) {
+            if (!__any(d < run)) break;
+            step(d);
         }
     }
     return true;
@@ -303,13 +375,13 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
                 for (int j = k >> 1; j > 0; j >>= 1) {
-                    const uint32_t oc = __shfl_xor(c, j, G);
+                    const uint32_t oc = xor_u32(c, j, G);
                     const bool up = (lane & k) == 0;
                     const bool lower = (lane & j) == 0;
                     if ((lower == up) ? (oc < c) : (oc > c)) c = oc;
                 }
             }
-            const uint32_t pc = __shfl_up(c, 1, G);
+            const uint32_t pc = up1_u32<G>(c);
             const int wl = threadIdx.x & 63;
             const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << (wl - lane));
             const uint64_t heads = __ballot(lane < n && (lane == 0 || pc != c));
