@@ -221,15 +221,21 @@ __device__ __forceinline__ void small_row_keys(const ListPool& p, int64_t a, int
     }
 }
 
-// After small_row_keys.  Returns false for a row longer than G.
+// After small_row_keys.  Returns false for a row longer than G.  `stage`
+// holds B values per thread of the block (the caller's LDS): each lane parks
+// its entry's values there, the keys are sorted with their original slots,
+// and the head lane of each run of equal columns reads the run's values by
+// slot and adds them one by one, 0 + v1 + v2 + ... (the serial order).
 template <int G, int B, typename VT>
 __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L,
-                                                SmallRow<G, B, VT>& r) {
+                                                SmallRow<G, B, VT>& r, VT* stage) {
     const int32_t n = L.n;
     if (n > G) return false;
     const int2 kh = L.kh;
     const int wl = threadIdx.x & 63;
     const int gbase = wl - lane;
+#pragma unroll
+    for (int b = 0; b < B; ++b) stage[threadIdx.x * B + b] = L.vh[b];
     const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
     uint64_t key = KEY_INF;
     int32_t slot = lane;
@@ -257,34 +263,30 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
     r.u = __popcll(gheads);
     const uint64_t le = wl == 63 ? ~0ull : ((1ull << (wl + 1)) - 1);
     r.rank = __popcll(gheads & le) - 1;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-        const double x = __shfl((double)L.vh[b], slot, G);   // values follow their keys
-        r.v[b] = valid ? x : 0.0;
-    }
     const uint64_t above = wl == 63 ? 0ull : (gheads & (~0ull << (wl + 1)));
     const int32_t run = r.head ? (above ? (__ffsll((unsigned long long)above) - 1 - gbase) : n) - lane : 0;
-    // Run sums: the head lane adds the run's values one by one.  Unrolled for
-    // 16-lane rows (DPP needs constant distances).
-    auto step = [&](int d) {
-        double x[B];
+    const VT* gstage = stage + (threadIdx.x - lane) * B;   // the group's values, by original slot
 #pragma unroll
-        for (int b = 0; b < B; ++b) x[b] = down_f64<G>(r.v[b], d);
-        if (d < run) {
+    for (int b = 0; b < B; ++b) r.v[b] = 0.0;
+    auto step = [&](int32_t sl) {
 #pragma unroll
-            for (int b = 0; b < B; ++b) r.v[b] += x[b];
-        }
+        for (int b = 0; b < B; ++b) r.v[b] += (double)gstage[sl * B + b];
     };
+    if (run > 0) step(slot);
+    // Slots of the run's other members: sorted positions lane+1, lane+2, ...
+    // Unrolled for 16-lane rows (DPP needs constant distances).
     if constexpr (G == 16) {
 #pragma unroll
         for (int d = 1; d < G; ++d) {
             if (!__any(d < run)) break;
-            step(d);
+            const int32_t sd = (int32_t)down_u32<G>((uint32_t)slot, d);
+            if (d < run) step(sd);
         }
     } else {
         for (int d = 1; d < G; ++d) {
             if (!__any(d < run)) break;
-            step(d);
+            const int32_t sd = (int32_t)down_u32<G>((uint32_t)slot, d);
+            if (d < run) step(sd);
         }
     }
     return true;
@@ -388,6 +390,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             if (lane == 0) u[a] = __popcll(heads & gmask);
         }
     } else {
+        __shared__ VT vstage[256 * B];
         SmallLoad<B, VT> L[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             const int64_t a = wbase + r * GPW;
             if (a >= p.n_nodes) break;
             SmallRow<G, B, VT> row;
-            if (!small_row_merge<G, B, VT>(p, a, lane, L[r], row)) continue;   // long row: other kernel
+            if (!small_row_merge<G, B, VT>(p, a, lane, L[r], row, vstage)) continue;   // long row: other kernel
             if (row.head) {
                 int64_t rs[3];
                 for (int dr = 0; dr < dpn; ++dr) rs[dr] = (int64_t)row_ptr[a * dpn + dr];
@@ -451,6 +454,7 @@ __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, 
     const int rib = threadIdx.x / G, lane = threadIdx.x % G;
     SmallRow<G, B, double> r[FUSED_ITER];
     SmallLoad<B, double> L[FUSED_ITER];
+    __shared__ double vstage[256 * B];
 #pragma unroll
     for (int it = 0; it < FUSED_ITER; ++it) {
         const int64_t a = tile * RPT + it * RPI + rib;
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, 
         const int64_t a = tile * RPT + it * RPI + rib;
         r[it].u = 0;
         r[it].head = false;
-        if (a < p.n_nodes && !small_row_merge<G, B, double>(p, a, lane, L[it], r[it])) {
+        if (a < p.n_nodes && !small_row_merge<G, B, double>(p, a, lane, L[it], r[it], vstage)) {
             // the host checked that no row is longer than G; never taken
             if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], G + 1);
             r[it].u = 0;
