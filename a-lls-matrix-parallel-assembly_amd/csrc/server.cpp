@@ -153,12 +153,17 @@ struct alls_server {
     double cost[2][3] = {{0.7, 1.2, 0.35}, {35.0, 120.0, 30.0}};
 
     enum { PATH_PLAIN = 0, PATH_BUILD = 1, PATH_HIT = 2 };
-    // Whether a request of a key seen `prior` times should build its pattern.
+    // Whether a request of a key seen `prior` times should build its pattern:
+    // only a key that has come back (prior >= 1), and only when its expected
+    // repeats (as many as it has had) pay for the build.  Under concurrency
+    // the measured build and plain costs come out close, so without the
+    // first condition every new key would build; in the served mix most keys
+    // never repeat and their patterns were evicted unused.
     bool worth_building(const Request& q, int prior) const {
         if (cfg.cache_policy == ALLS_CACHE_EAGER) return true;
+        if (prior < 1) return false;
         const double* c = cost[q.mesh.dim == 3 ? 1 : 0];
-        const int repeats = prior > 1 ? prior : 1;   // a keyed request is expected to repeat at least once
-        return repeats * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
+        return prior * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
     }
     // Small plain requests queued right behind a plain head join it in one
     // packed batch (one pipeline, persistent insertion, split afterwards).

@@ -227,10 +227,10 @@ alls_status alls_autotune(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
  * the budget, run concurrently on up to n_streams streams, and reuse cached
  * patterns when mesh_key repeats. */
 /* When a request with a mesh key builds and caches its pattern.  COST (the
- * default): when the expected repeats (the key's earlier requests, at least
- * one) times the saving of a cached repeat exceed the extra cost of the
- * build, with per-element costs the server measures per path and dimension.
- * EAGER: on the first request of every key. */
+ * default): once the key has come back, when the expected repeats (the
+ * key's earlier requests) times the saving of a cached repeat exceed the
+ * extra cost of the build, with per-element costs the server measures per
+ * path and dimension.  EAGER: on the first request of every key. */
 typedef enum { ALLS_CACHE_COST = 0, ALLS_CACHE_EAGER = 1 } alls_cache_policy;
 
 typedef struct {

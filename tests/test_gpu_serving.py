@@ -66,8 +66,8 @@ def test_cost_policy_builds_only_when_repeats_pay_for_it():
     # Default policy with the seeded 2D costs (plain 0.7, build 1.2, hit 0.35
     # ns/element; small requests do not update them): a key builds its
     # pattern once 2 earlier requests predict enough repeats (2 * 0.35 > 0.5;
-    # a new key counts as one repeat), and hits after that.  Every result
-    # equals the plain assembly.
+    # a key seen for the first time never builds), and hits after that.
+    # Every result equals the plain assembly.
     m = alls.generate(alls.GEN_Q4, 30, 20, jitter=0.2, seed=5)
     rng = np.random.default_rng(3)
     kinds, outs = [], []
