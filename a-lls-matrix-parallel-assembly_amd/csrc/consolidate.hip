@@ -812,9 +812,10 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
                                                        int32_t* col_ind, double* val) {
     constexpr int D = B == 4 ? 2 : 3;
     constexpr int CAP = 64 * EPL;
-    constexpr int TS = 2 * CAP;
-    __shared__ __align__(16) int32_t sm[11 * CAP + 1];
-    // Phase arrays; the first 8*CAP ints are dead once the buckets are sorted
+    constexpr int TS = 2 * CAP <= 128 ? 128 : 2 * CAP <= 256 ? 256 : 2 * CAP <= 512 ? 512 : 1024;   // power of two
+    constexpr int PH = TS + 6 * CAP;   // phase arrays
+    __shared__ __align__(16) int32_t sm[PH + 3 * CAP + 1];
+    // Phase arrays; the first PH ints are dead once the buckets are sorted
     // and then hold the value stage (CAP * D doubles).
     int32_t* table = sm;               // [TS]
     int32_t* eeid = table + TS;        // [CAP]
@@ -824,10 +825,10 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
     int32_t* urank = ucnt + CAP;       // [CAP]
     int32_t* gfill = urank + CAP;      // [CAP]
     double* stage = reinterpret_cast<double*>(sm);
-    int32_t* eu = sm + 8 * CAP;        // [CAP]  slot, unique index, then bucket position of entry i
+    int32_t* eu = sm + PH;             // [CAP]  slot, unique index, then bucket position of entry i
     int32_t* goff = eu + CAP;          // [CAP+1]
     int32_t* rcol = goff + CAP + 1;    // [CAP]
-    static_assert(CAP * D * 2 <= 8 * CAP, "value stage must fit the dead phase arrays");
+    static_assert(CAP * D * 2 <= PH, "value stage must fit the dead phase arrays");
     const int lane = threadIdx.x;
     constexpr int shift = 32 - (31 - __builtin_clz(TS));
     const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
@@ -1174,11 +1175,17 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
     while (cap < max_big) cap <<= 1;
     const int32_t* bl_or_all = G > 0 ? big_list : nullptr;
     if (cap <= 256) {
-        if (cap == 64)
+        // Entries per lane: the fewer, the less LDS and registers per wave
+        // (Hex20 rows of at most 160 entries take 3: 4 -> 5 waves/SIMD).
+        const int epl = (max_big + 63) / 64;
+        if (epl == 1)
             hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
                                rp, ci, v);
-        else if (cap == 128)
+        else if (epl == 2)
             hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 2>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+                               rp, ci, v);
+        else if (epl == 3)
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 3>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
                                rp, ci, v);
         else
             hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,

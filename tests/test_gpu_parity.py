@@ -56,6 +56,29 @@ def test_hex20_f32_mfma_frobenius(ctx):
     assert fro <= 1e-6
 
 
+@pytest.mark.parametrize("kind,kw", [
+    (alls.GEN_Q4, dict(nx=7, ny=6, jitter=0.1, seed=3)),
+    (alls.GEN_T3, dict(nx=7, ny=6, jitter=0.1, seed=3, random_diagonal=1)),
+    (alls.GEN_MIXED, dict(nx=7, ny=6, jitter=0.1, seed=3, random_diagonal=1)),
+    (alls.GEN_HEX20, dict(nx=3, ny=2, nz=2, jitter=0.1, seed=3)),
+], ids=["q4", "t3", "mixed", "hex20"])
+def test_element_with_a_repeated_node(ctx, kind, kw):
+    # A degenerate element (one node id twice) puts two contributions of one
+    # element on the same (row, column): the list layouts must keep both (the
+    # Hex20 group merge falls back to scanning its groups for such a row).
+    m = alls.generate(kind, **kw)
+    conn = m.conn.reshape(m.n_elems, m.npe).copy()
+    e = m.n_elems // 2
+    conn[e, 1] = conn[e, 0]
+    m = alls.MeshArrays(m.coords, conn, m.dim, m.npe, m.uniform_type, m.elem_type, m.E, m.E_per_elem, m.nu,
+                        m.thickness)
+    print(f"repeated node kind {kind} element {e}", flush=True)
+    out = ctx.assemble(m)
+    gpu = out.download()
+    out.free()
+    compare_csr(gpu, oracle.assemble(m))
+
+
 def test_repeat_runs_are_bitwise_identical(ctx):
     m = alls.generate(alls.GEN_T3, 80, 80, jitter=0.3, seed=3, random_diagonal=1, permute_nodes=1)
     outs = []
