@@ -1018,7 +1018,11 @@ constexpr int LONG_CAP_MAX = 4096;    // k_rows_big: 40 KB of LDS
 // per row at a time, `ts` table slots, ts >= 2 * ALLS_DIR * C).  Counting
 // needs no order, so no sort: each entry inserts its column with one LDS
 // compare-and-swap (linear probing).
-__global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_t* list, int32_t* u, int ts) {
+// With `defer` set, rows that need more than `ts` slots are appended to it
+// (count in MISC_BIG_COUNT) for a second launch with a larger table, so the
+// common short rows run with a small LDS allocation and more waves per CU.
+__global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_t* list, int32_t* u, int ts,
+                                                        int32_t* defer) {
     extern __shared__ int32_t table[];
     const int lane = threadIdx.x;
     const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
@@ -1037,6 +1041,10 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
         // A table of at least twice the row's entries (power of two, <= ts).
         int tsr = 128;
         while (tsr < 2 * n && tsr < ts) tsr <<= 1;
+        if (defer && tsr < 2 * n) {
+            if (lane == 0) defer[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
+            continue;
+        }
         const int shift = 32 - (31 - __clz(tsr));
         for (int i = lane; i < tsr; i += 64) table[i] = -1;
         __syncthreads();
@@ -1166,8 +1174,18 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
         int cap = 1;
         while (cap < ALLS_DIR * p.C) cap <<= 1;
         const int ts = 2 * cap;
-        hipLaunchKernelGGL(k_rows_hash_count, dim3((unsigned)grid), dim3(64), (size_t)ts * sizeof(int32_t), s, p,
-                           G > 0 ? big_list : nullptr, u, ts);
+        if (G > 0 || ts <= 512) {
+            hipLaunchKernelGGL(k_rows_hash_count, dim3((unsigned)grid), dim3(64), (size_t)ts * sizeof(int32_t), s,
+                               p, G > 0 ? big_list : nullptr, u, ts, nullptr);
+            return;
+        }
+        // Every row (Hex20): a 512-slot table for rows of up to 256 entries
+        // (2 KB of LDS instead of 8 KB at C = 80: 5 -> 8 waves/SIMD), then
+        // the rows beyond it, listed in big_list, with the full table.
+        hipLaunchKernelGGL(k_rows_hash_count, dim3((unsigned)grid), dim3(64), 512 * sizeof(int32_t), s, p,
+                           nullptr, u, 512, bl);
+        hipLaunchKernelGGL(k_rows_hash_count, dim3(2048), dim3(64), (size_t)ts * sizeof(int32_t), s, p, bl, u, ts,
+                           nullptr);
         return;
     }
     if (max_big <= 0) return;   // no long rows

@@ -79,6 +79,18 @@ def test_element_with_a_repeated_node(ctx, kind, kw):
     compare_csr(gpu, oracle.assemble(m))
 
 
+def test_hex20_rows_beyond_the_short_count_table(ctx):
+    # Every element listed twice: vertex rows get 320 entries, more than the
+    # first count launch's table holds, so they are counted by the second
+    # launch and merged by the LDS kernel.
+    m = alls.generate(alls.GEN_HEX20, 2, 2, 2, jitter=0.1, seed=9)
+    conn = np.concatenate([m.conn.reshape(m.n_elems, 20)] * 2)
+    m = alls.MeshArrays(m.coords, conn, 3, 20, m.uniform_type, None, m.E, None, m.nu, m.thickness)
+    out = ctx.assemble(m)
+    assert ctx.stats()["big_rows"] >= 1
+    compare_csr(out.download(), oracle.assemble(m))
+
+
 def test_repeat_runs_are_bitwise_identical(ctx):
     m = alls.generate(alls.GEN_T3, 80, 80, jitter=0.3, seed=3, random_diagonal=1, permute_nodes=1)
     outs = []
