@@ -67,6 +67,15 @@ def offered_busy_s(reqs, costs=None):
     return total / 1e3
 
 
+def one_stream_ms(server, reqs):
+    """Mean wall time (ms) per request when requests are served one at a
+    time (submit, wait): the measured one-stream service time of the mix."""
+    t0 = time.perf_counter()
+    for mid, m in reqs:
+        server.wait(server.submit(m, key=mid + 1), keep_output=False)
+    return (time.perf_counter() - t0) * 1e3 / max(len(reqs), 1)
+
+
 def run(server, reqs, rate_per_s, seed=1, progress=None):
     """Submits reqs at Poisson arrival times; returns per-request stats."""
     rng = np.random.default_rng(seed)
