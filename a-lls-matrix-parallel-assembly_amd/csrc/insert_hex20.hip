@@ -99,6 +99,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
     __shared__ double xyz[60];
     __shared__ int32_t nodes[20];
     __shared__ int32_t posv[20];
+    __shared__ int64_t rbase[20];   // list index of each row group's first slot (-1: spans chunks, -2: none)
     __shared__ int32_t bad;
 
     const int lane = threadIdx.x;
@@ -226,6 +227,15 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                 const int32_t pos = list_reserve(p, nodes[lane], 20);
                 list_own_chunks(p, nodes[lane], pos, 20);
                 posv[lane] = pos;
+                // The group's chunk, found once per row instead of once per
+                // entry (an overflow chunk's directory word is an L2 access).
+                const int32_t k0 = chunk_of(p, pos);
+                int64_t b = -1;
+                if (k0 == chunk_of(p, pos + 19)) {
+                    const int64_t ch = list_chunk(p, nodes[lane], pos);
+                    b = ch < 0 ? -2 : ch * p.C + (pos - k0 * p.C);
+                }
+                rbase[lane] = b;
             }
         }
         __syncthreads();
@@ -244,10 +254,16 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
     #pragma unroll
                 for (int c = 0; c < 3; ++c) kb[r][c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
             if (EMIT == 0) {
-                const int32_t s = posv[I] + Jn;
-                const int64_t ch = list_chunk(p, nodes[I], s);
-                if (ch < 0) continue;
-                const int64_t id = ch * p.C + (s - chunk_of(p, s) * p.C);
+                int64_t id = rbase[I];
+                if (id == -2) continue;
+                if (id >= 0) {
+                    id += Jn;
+                } else {   // the group spans two chunks (C not a multiple of 20)
+                    const int32_t s = posv[I] + Jn;
+                    const int64_t ch = list_chunk(p, nodes[I], s);
+                    if (ch < 0) continue;
+                    id = ch * p.C + (s - chunk_of(p, s) * p.C);
+                }
                 T* dst = static_cast<T*>(p.vals) + id * 9;
     #pragma unroll
                 for (int r = 0; r < 3; ++r)
