@@ -225,19 +225,6 @@ ALLS_DEV void list_store_group(const ListPool& p, int32_t a, int32_t pos, int32_
     p.geid[ch * p.gpc + group_of(p, r)] = e;
 }
 
-// Q4 / T3: the key of slot r (any slot of the row with the same r % ng)
-// given its group's element e: the column node is conn[e][r % ng].  An
-// element id out of range (a group left unwritten after a failed chunk wait,
-// which the host then retries) reads as column a with *pad set (its value
-// counts as zero), so no access leaves the arrays.
-ALLS_DEV int2 group_key(const ListPool& p, int64_t a, int32_t r, int32_t e, bool* pad) {
-    const int32_t q = r - group_of(p, r) * p.ng;
-    const bool bad = (uint64_t)(uint32_t)e >= (uint64_t)p.n_elems;
-    const int32_t b = bad ? -1 : p.conn[(int64_t)e * p.npe + q];
-    *pad = b < 0;
-    return make_int2(b < 0 ? (int32_t)a : b, bad ? 0 : e);
-}
-
 #endif  // __HIPCC__
 
 // ----------------------------------------------------------- host side ---

@@ -99,6 +99,25 @@ __device__ __forceinline__ int64_t entry_index(const ListPool& p, int64_t a, con
     return ch * p.C + (s - k * p.C);
 }
 
+// Group layout, with the group size NG (3 or 4) known at compile time (NG = 0
+// means per-entry keys).  gkey: the key of slot r (any slot of the row with
+// the same r % NG) given its group's element e: the column node is
+// conn[e][r % NG].  An element id out of range (a group left unwritten after
+// a failed chunk wait, which the host then retries) reads as column a with
+// *pad set (its value counts as zero), so no access leaves the arrays.
+template <int NG>
+__device__ __forceinline__ int32_t grp(int32_t r) {
+    return NG == 4 ? (r >> 2) : (int32_t)(__umulhi((uint32_t)r, 0xAAAAAAABu) >> 1);
+}
+template <int NG>
+__device__ __forceinline__ int2 gkey(const ListPool& p, int64_t a, int32_t r, int32_t e, bool* pad) {
+    const int32_t q = r - grp<NG>(r) * NG;
+    const bool bad = (uint64_t)(uint32_t)e >= (uint64_t)p.n_elems;
+    const int32_t b = bad ? -1 : p.conn[(int64_t)e * p.npe + q];
+    *pad = b < 0;
+    return make_int2(b < 0 ? (int32_t)a : b, bad ? 0 : e);
+}
+
 // Chunk and in-chunk slot of slot s of row a (as entry_index).
 __device__ __forceinline__ void entry_loc(const ListPool& p, int64_t a, const int32_t* h, int32_t s, int64_t* ch,
                                           int32_t* r) {
@@ -112,7 +131,7 @@ __device__ __forceinline__ void entry_loc(const ListPool& p, int64_t a, const in
 // kernels will take (more than G entries), from the group element ids, and
 // zeroes the value of each unused triangle slot (read as column a).  Lanes
 // 0..G-1 of the row's group take slots lane, lane+G, ...
-template <int B, typename VT>
+template <int NG, int B, typename VT>
 __device__ __forceinline__ void materialize_keys(const ListPool& p, int64_t a, int lane, int G, int32_t n) {
     const int32_t* h = p.hdr + a * ALLS_DIR;
     for (int32_t sl = lane; sl < n; sl += G) {
@@ -120,7 +139,7 @@ __device__ __forceinline__ void materialize_keys(const ListPool& p, int64_t a, i
         int32_t r;
         entry_loc(p, a, h, sl, &ch, &r);
         bool pad;
-        const int2 key = group_key(p, a, r, p.geid[ch * p.gpc + group_of(p, r)], &pad);
+        const int2 key = gkey<NG>(p, a, r, p.geid[ch * p.gpc + grp<NG>(r)], &pad);
         const int64_t idx = ch * p.C + r;
         p.keys[idx] = key;
         if (pad) {
@@ -175,11 +194,11 @@ struct SmallLoad {
     VT vh[B];
 };
 
-template <int B, typename VT, bool VALS>
+template <int NG, int B, typename VT, bool VALS>
 __device__ __forceinline__ void small_row_load(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L) {
     L.kh = make_int2(0, 0);
     if (lane < p.C) {
-        if (p.ng) L.kh.y = p.geid[a * p.gpc + group_of(p, lane)];   // 2D: the group's element
+        if (NG) L.kh.y = p.geid[a * p.gpc + grp<NG>(lane)];   // group layout: the group's element
         else L.kh = p.keys[a * p.C + lane];
         if constexpr (VALS) {
             const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + lane) * B;
@@ -195,7 +214,7 @@ __device__ __forceinline__ void small_row_load(const ListPool& p, int64_t a, int
 // column from its group's element.  Rows are resolved a few at a time before
 // any is merged, so that these dependent loads overlap too.  A row longer
 // than G is left alone (the wave-per-row kernels take it).
-template <int G, int B, typename VT>
+template <int G, int NG, int B, typename VT>
 __device__ __forceinline__ void small_row_keys(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L) {
     const int32_t n = min(L.n, ALLS_DIR * p.C);
     L.n = n;
@@ -205,15 +224,15 @@ __device__ __forceinline__ void small_row_keys(const ListPool& p, int64_t a, int
         int32_t r;
         entry_loc(p, a, p.hdr + a * ALLS_DIR, lane, &ch, &r);
         const int64_t ei = ch * p.C + r;
-        if (p.ng) L.kh.y = p.geid[ch * p.gpc + group_of(p, r)];
+        if (NG) L.kh.y = p.geid[ch * p.gpc + grp<NG>(r)];
         else L.kh = p.keys[ei];
         const VT* src = static_cast<const VT*>(p.vals) + ei * B;
 #pragma unroll
         for (int b = 0; b < B; ++b) L.vh[b] = src[b];
     }
-    if (p.ng && lane < n) {   // 2D: column from the group's element
+    if (NG && lane < n) {   // group layout: column from the group's element
         bool pad;
-        L.kh = group_key(p, a, lane, L.kh.y, &pad);
+        L.kh = gkey<NG>(p, a, lane, L.kh.y, &pad);
         if (pad) {
 #pragma unroll
             for (int b = 0; b < B; ++b) L.vh[b] = VT(0);
@@ -321,7 +340,7 @@ __device__ __forceinline__ void small_row_store(int32_t col, int32_t rank, const
 // loads of all R rounds are issued before the first row is merged.  Row of
 // round r: wave base + r * 64/G + group, so that each round reads and writes
 // one contiguous stretch as a single round would.
-template <int G, int R, int B, typename VT, bool WRITE, typename RP>
+template <int G, int R, int NG, int B, typename VT, bool WRITE, typename RP>
 __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int32_t* big_list,
                                                     const RP* row_ptr, int32_t* col_ind, double* val,
                                                     int dpn) {
@@ -341,7 +360,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             hn[r] = 0;
             if (a < p.n_nodes) {
                 if (lane < p.C)   // with the header, not after it
-                    kc[r] = p.ng ? p.geid[a * p.gpc + group_of(p, lane)] : p.keys[a * p.C + lane].x;
+                    kc[r] = NG ? p.geid[a * p.gpc + grp<NG>(lane)] : p.keys[a * p.C + lane].x;
                 hn[r] = min(p.hdr[a * ALLS_DIR], ALLS_DIR * p.C);
             }
         }
@@ -355,11 +374,11 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
                 int64_t ch;
                 int32_t rr;
                 entry_loc(p, a, p.hdr + a * ALLS_DIR, lane, &ch, &rr);
-                kc[r] = p.ng ? p.geid[ch * p.gpc + group_of(p, rr)] : p.keys[ch * p.C + rr].x;
+                kc[r] = NG ? p.geid[ch * p.gpc + grp<NG>(rr)] : p.keys[ch * p.C + rr].x;
             }
-            if (p.ng && lane < n) {
+            if (NG && lane < n) {
                 bool pad;
-                kc[r] = group_key(p, a, lane, kc[r], &pad).x;
+                kc[r] = gkey<NG>(p, a, lane, kc[r], &pad).x;
             }
         }
 #pragma unroll
@@ -369,7 +388,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             const int32_t n = hn[r];
             if (n > G) {
                 if (lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
-                if (p.ng) materialize_keys<B, VT>(p, a, lane, G, n);
+                if (NG) materialize_keys<NG, B, VT>(p, a, lane, G, n);
                 continue;
             }
             uint32_t c = lane < n ? (uint32_t)kc[r] : 0xffffffffu;
@@ -394,10 +413,10 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
         SmallLoad<B, VT> L[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (wbase + r * GPW < p.n_nodes) small_row_load<B, VT, true>(p, wbase + r * GPW, lane, L[r]);
+            if (wbase + r * GPW < p.n_nodes) small_row_load<NG, B, VT, true>(p, wbase + r * GPW, lane, L[r]);
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (wbase + r * GPW < p.n_nodes) small_row_keys<G, B, VT>(p, wbase + r * GPW, lane, L[r]);
+            if (wbase + r * GPW < p.n_nodes) small_row_keys<G, NG, B, VT>(p, wbase + r * GPW, lane, L[r]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int64_t a = wbase + r * GPW;
@@ -437,7 +456,7 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t x) {
     return x;
 }
 
-template <int G, typename RP>
+template <int G, int NG, typename RP>
 __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, int64_t n_tiles, RP* row_ptr,
                                                     int32_t* col_ind, double* val) {
     constexpr int RPI = 256 / G;                 // rows per iteration
@@ -458,12 +477,12 @@ __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, 
 #pragma unroll
     for (int it = 0; it < FUSED_ITER; ++it) {
         const int64_t a = tile * RPT + it * RPI + rib;
-        if (a < p.n_nodes) small_row_load<B, double, true>(p, a, lane, L[it]);
+        if (a < p.n_nodes) small_row_load<NG, B, double, true>(p, a, lane, L[it]);
     }
 #pragma unroll
     for (int it = 0; it < FUSED_ITER; ++it) {
         const int64_t a = tile * RPT + it * RPI + rib;
-        if (a < p.n_nodes) small_row_keys<G, B, double>(p, a, lane, L[it]);
+        if (a < p.n_nodes) small_row_keys<G, NG, B, double>(p, a, lane, L[it]);
     }
 #pragma unroll
     for (int it = 0; it < FUSED_ITER; ++it) {
@@ -1159,12 +1178,26 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
         const int64_t rows_per_wave = (int64_t)(64 / G) * R;
         const int64_t threads = (p.n_nodes + rows_per_wave - 1) / rows_per_wave * 64;
         const unsigned blocks = (unsigned)((threads + 255) / 256);
-        if (G == 16)
-            hipLaunchKernelGGL((k_rows_small<16, R, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn);
-        else if (G == 32)
-            hipLaunchKernelGGL((k_rows_small<32, R, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn);
-        else
-            hipLaunchKernelGGL((k_rows_small<64, R, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn);
+#define ALLS_SMALL(GG, NGG) \
+        hipLaunchKernelGGL((k_rows_small<GG, R, NGG, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn)
+        if constexpr (B == 4) {   // 2D: Q4 groups of 4, T3 groups of 3, mixed keys
+            if (G == 16) {
+                if (p.ng == 4) ALLS_SMALL(16, 4);
+                else if (p.ng == 3) ALLS_SMALL(16, 3);
+                else ALLS_SMALL(16, 0);
+            } else if (G == 32) {
+                if (p.ng == 4) ALLS_SMALL(32, 4);
+                else if (p.ng == 3) ALLS_SMALL(32, 3);
+                else ALLS_SMALL(32, 0);
+            } else {
+                ALLS_SMALL(64, 0);
+            }
+        } else {
+            if (G == 16) ALLS_SMALL(16, 0);
+            else if (G == 32) ALLS_SMALL(32, 0);
+            else ALLS_SMALL(64, 0);
+        }
+#undef ALLS_SMALL
     }
     // Long rows: over the big-row list (G > 0) or over every row (G == 0).
     int64_t grid = G > 0 ? 2048 : p.n_nodes;
@@ -1239,16 +1272,23 @@ void launch_rows_fused(const ListPool& p, int G, int64_t* tstat, void* row_ptr, 
                        double* val, hipStream_t s) {
     const int64_t nt = fused_tile_count(p.n_nodes, G);
     if (nt <= 0) return;
-#define ALLS_FU(GG, RP)                                                                                    \
-    hipLaunchKernelGGL((k_rows_fused<GG, RP>), dim3((unsigned)nt), dim3(256), 0, s, p, tstat, nt,            \
+#define ALLS_FU(GG, NGG, RP)                                                                               \
+    hipLaunchKernelGGL((k_rows_fused<GG, NGG, RP>), dim3((unsigned)nt), dim3(256), 0, s, p, tstat, nt,       \
                        static_cast<RP*>(row_ptr), col_ind, val)
+#define ALLS_FU_NG(GG, RP)                     \
+    do {                                       \
+        if (p.ng == 4) ALLS_FU(GG, 4, RP);     \
+        else if (p.ng == 3) ALLS_FU(GG, 3, RP); \
+        else ALLS_FU(GG, 0, RP);               \
+    } while (0)
     if (G == 16) {
-        if (i64) ALLS_FU(16, int64_t);
-        else ALLS_FU(16, int32_t);
+        if (i64) ALLS_FU_NG(16, int64_t);
+        else ALLS_FU_NG(16, int32_t);
     } else {
-        if (i64) ALLS_FU(32, int64_t);
-        else ALLS_FU(32, int32_t);
+        if (i64) ALLS_FU_NG(32, int64_t);
+        else ALLS_FU_NG(32, int32_t);
     }
+#undef ALLS_FU_NG
 #undef ALLS_FU
 }
 
