@@ -111,11 +111,13 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                     if (ch >= 0) {
                         const int64_t idx = ch * p.C + (s - chunk_of(p, s) * p.C);
                         // Non-temporal: the lists are read once, by another
-                        // kernel, and whole 128-B groups are written here
-                        // (0.58 -> 0.54 ms at Q4 2000x2000; the count pass
-                        // after it also gained 6%).
+                        // kernel (Q4 2000x2000: 0.58 -> 0.54 ms, and the
+                        // count pass after it gained 6%; mixed 1500x1500:
+                        // 0.86 -> 0.84 ms).  Not for T3, whose 96-B groups
+                        // share cache lines: 0.78 -> 0.95 ms that way.
                         const double2 x = st.v[src][w];
-                        __builtin_nontemporal_store(f64x2{x.x, x.y}, reinterpret_cast<f64x2*>(p.vals) + idx * 2 + (w & 1));
+                        if (MODE == 1) reinterpret_cast<double2*>(p.vals)[idx * 2 + (w & 1)] = x;
+                        else __builtin_nontemporal_store(f64x2{x.x, x.y}, reinterpret_cast<f64x2*>(p.vals) + idx * 2 + (w & 1));
                     }
                 }
             }
