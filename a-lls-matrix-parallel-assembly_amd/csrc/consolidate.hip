@@ -433,7 +433,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
 }
 
 // Count, scan and write of small rows in ONE pass (2D meshes whose rows all
-// fit G lanes).  Blocks take tiles of FUSED_ITER * 256/G rows in the order
+// fit G lanes).  Blocks take tiles of fused_iter(G) * 256/G rows in the order
 // they start (ticket from an atomic counter), merge their rows as the write
 // pass does (results kept in registers), and find the rows' CSR offsets with
 // a decoupled look-back over the tiles started before them: each tile
@@ -448,7 +448,10 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
 // device scope would write back or invalidate the XCD's caches per access.
 constexpr int64_t TILE_AGG = int64_t(1) << 62, TILE_PREFIX = int64_t(2) << 62;
 constexpr int64_t TILE_VALUE = TILE_AGG - 1;
-constexpr int FUSED_ITER = 4;
+// Rows per lane group in a tile: 6 for 32-lane rows (48 rows per tile;
+// measured against 2, 4 and 8: T3 1.38 -> 1.35 ms, mixed 1.11 -> 1.04 ms at
+// 1500x1500), 4 for 16-lane rows (a tile's rows must fit one wave's scan).
+__host__ __device__ constexpr int fused_iter(int G) { return G == 32 ? 6 : 4; }
 
 __device__ __forceinline__ int64_t wave_sum64(int64_t x) {
 #pragma unroll
@@ -460,6 +463,7 @@ template <int G, int NG, typename RP>
 __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, int64_t n_tiles, RP* row_ptr,
                                                     int32_t* col_ind, double* val) {
     constexpr int RPI = 256 / G;                 // rows per iteration
+    constexpr int FUSED_ITER = fused_iter(G);
     constexpr int RPT = RPI * FUSED_ITER;        // rows per tile (<= 64)
     constexpr int B = 4, D = 2;
     static_assert(RPT <= 64, "one wave scans a tile's rows");
@@ -1264,7 +1268,7 @@ int32_t long_row_limit() { return LONG_CAP_MAX; }
 int64_t scan_tile_count(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 int64_t fused_tile_count(int64_t n_nodes, int G) {
-    const int64_t rpt = (int64_t)(256 / G) * FUSED_ITER;
+    const int64_t rpt = (int64_t)(256 / G) * fused_iter(G);
     return (n_nodes + rpt - 1) / rpt;
 }
 
