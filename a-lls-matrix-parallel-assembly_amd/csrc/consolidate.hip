@@ -432,6 +432,68 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
     }
 }
 
+// Count pass for Q4 lists (groups of 4 slots): a lane per group, 4 lanes
+// (a DPP quad) per row, so a wave counts 16 rows.  Each lane loads its
+// group's element id and the element's 4 nodes (one 16-byte load); a column
+// counts if no lower lane of the quad holds it and no earlier column of its
+// own lane equals it, and the quad adds its counts.  Rows of more than 16
+// entries go to the wave-per-row kernels as in k_rows_small.
+template <int B, typename VT>
+__global__ __launch_bounds__(256) void k_count_quad(ListPool p, int32_t* u, int32_t* big_list) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t a = gid >> 2;
+    const int g = (int)(gid & 3);
+    if (a >= p.n_nodes) return;   // whole quads
+    const int32_t* h = p.hdr + a * ALLS_DIR;
+    int32_t e = 4 * g < p.C ? p.geid[a * p.gpc + g] : -1;   // with the header, not after it
+    const int32_t n = min(h[0], ALLS_DIR * p.C);
+    if (n > 16) {
+        if (g == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
+        materialize_keys<4, B, VT>(p, a, g, 4, n);
+        return;
+    }
+    const bool valid = 4 * g < n;
+    if (valid && 4 * g >= p.C) {   // overflow chunk (chunks of fewer than 16 slots)
+        int64_t ch;
+        int32_t r;
+        entry_loc(p, a, h, 4 * g, &ch, &r);
+        e = p.geid[ch * p.gpc + grp<4>(r)];
+    }
+    int32_t v[4] = {-1, -1, -1, -1};
+    if (valid) {
+        if ((uint64_t)(uint32_t)e < (uint64_t)p.n_elems) {
+            const int4 c = *reinterpret_cast<const int4*>(p.conn + (int64_t)e * 4);
+            v[0] = c.x; v[1] = c.y; v[2] = c.z; v[3] = c.w;
+        } else {   // a group left unwritten (the host retries): column a, as gkey reads it
+            v[0] = v[1] = v[2] = v[3] = (int32_t)a;
+        }
+    }
+    // Columns of the other lanes of the quad (DPP quad broadcasts).
+    int32_t w[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        w[0][j] = (int32_t)dpp_u32<0x00>((uint32_t)v[j]);   // quad_perm [0,0,0,0]
+        w[1][j] = (int32_t)dpp_u32<0x55>((uint32_t)v[j]);   // [1,1,1,1]
+        w[2][j] = (int32_t)dpp_u32<0xAA>((uint32_t)v[j]);   // [2,2,2,2]
+        w[3][j] = (int32_t)dpp_u32<0xFF>((uint32_t)v[j]);   // [3,3,3,3]
+    }
+    int32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        bool first = v[i] >= 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) first &= !(k < g && w[k][j] == v[i]);
+#pragma unroll
+        for (int j = 0; j < i; ++j) first &= v[j] != v[i];
+        cnt += first ? 1 : 0;
+    }
+    cnt += (int32_t)dpp_u32<0xB1>((uint32_t)cnt);   // quad_perm [1,0,3,2]
+    cnt += (int32_t)dpp_u32<0x4E>((uint32_t)cnt);   // quad_perm [2,3,0,1]
+    if (g == 0) u[a] = cnt;
+}
+
 // Count, scan and write of small rows in ONE pass (2D meshes whose rows all
 // fit G lanes).  Blocks take tiles of fused_iter(G) * 256/G rows in the order
 // they start (ticket from an atomic counter), merge their rows as the write
@@ -1182,6 +1244,13 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
         const int64_t rows_per_wave = (int64_t)(64 / G) * R;
         const int64_t threads = (p.n_nodes + rows_per_wave - 1) / rows_per_wave * 64;
         const unsigned blocks = (unsigned)((threads + 255) / 256);
+        if constexpr (B == 4 && !WRITE) {
+            if (p.ng == 4 && G == 16) {   // Q4 groups: a quad of lanes per row
+                const unsigned qb = (unsigned)((p.n_nodes * 4 + 255) / 256);
+                hipLaunchKernelGGL((k_count_quad<B, VT>), dim3(qb), dim3(256), 0, s, p, u, bl);
+                goto long_rows;
+            }
+        }
 #define ALLS_SMALL(GG, NGG) \
         hipLaunchKernelGGL((k_rows_small<GG, R, NGG, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn)
         if constexpr (B == 4) {   // 2D: Q4 groups of 4, T3 groups of 3, mixed keys
@@ -1203,6 +1272,7 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
         }
 #undef ALLS_SMALL
     }
+long_rows:
     // Long rows: over the big-row list (G > 0) or over every row (G == 0).
     int64_t grid = G > 0 ? 2048 : p.n_nodes;
     if (grid > 262144) grid = 262144;
