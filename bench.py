@@ -102,8 +102,8 @@ def roofline(mesh, st, val_bytes, key_bytes):
 
 
 # Kernel behind each pipeline stage of the Q4 workload (rocprofv3 names).
-STAGE_KERNEL = {"insert": "k_insert_2d<0>", "count": "k_rows_small<16, 2, 4, double, false, int>",
-                "write": "k_rows_small<16, 1, 4, double, true, int>"}
+STAGE_KERNEL = {"insert": "k_insert_2d<0>", "count": "k_count_quad<4, double>",
+                "write": "k_rows_small<16, 1, 4, 4, double, true, int>"}
 
 
 def measured_traffic(stage):
@@ -367,8 +367,8 @@ def served_mix(n_requests, costs):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=2000, help="Q4 grid size (workload n x n)")
     ap.add_argument("--cpu-n", type=int, default=1000, help="CPU baseline sample grid size")
     ap.add_argument("--hex-n", type=int, default=80)
