@@ -48,6 +48,7 @@ enum {
     MISC_TICKET = 12,          // tile tickets of the fused small-row pass
     MISC_TOTAL_LO = 13,        // unique node pairs found by the fused pass (low 31 bits)
     MISC_TOTAL_HI = 14,        //   and the bits above
+    MISC_DEFER = 15,           // rows the segment merge leaves to k_rows_merge_reg
     MISC_SIZE = 16
 };
 
@@ -303,7 +304,9 @@ void launch_scan_total(const int32_t* u, int64_t n, int64_t* tiles, int64_t ntil
 void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dpn, void* row_ptr,
                        int i64, hipStream_t s);
 // max_big: most entries in a row of the wave-per-row kernel (MISC_MAX_BIG).
-void launch_write(const ListPool& p, int small_group, int f32vals, int dpn, int32_t max_big, const int32_t* big_list,
+// big_list: the count pass's long-row list; when every row is long (3D
+// lists) the write pass reuses it for the rows it defers.
+void launch_write(const ListPool& p, int small_group, int f32vals, int dpn, int32_t max_big, int32_t* big_list,
                   const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s);
 int64_t scan_tile_count(int64_t n);
 // Fused count + scan + write of 2D meshes whose rows all hold <= G entries

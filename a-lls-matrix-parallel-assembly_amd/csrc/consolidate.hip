@@ -894,7 +894,7 @@ __global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* li
 // round trip per row instead of one per bucket member.
 template <int B, typename VT, typename RP, int EPL>
 __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t* list, const RP* row_ptr,
-                                                       int32_t* col_ind, double* val) {
+                                                       int32_t* col_ind, double* val, int count_slot) {
     constexpr int D = B == 4 ? 2 : 3;
     constexpr int CAP = 64 * EPL;
     constexpr int TS = 2 * CAP <= 128 ? 128 : 2 * CAP <= 256 ? 256 : 2 * CAP <= 512 ? 512 : 1024;   // power of two
@@ -916,7 +916,7 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
     static_assert(CAP * D * 2 <= PH, "value stage must fit the dead phase arrays");
     const int lane = threadIdx.x;
     constexpr int shift = 32 - (31 - __builtin_clz(TS));
-    const int64_t count = list ? (int64_t)p.misc[MISC_BIG_COUNT] : p.n_nodes;
+    const int64_t count = list ? (int64_t)p.misc[count_slot] : p.n_nodes;
     for (int64_t row = xcd_block(blockIdx.x, gridDim.x); row < count; row += gridDim.x) {
         const int64_t a = list ? (int64_t)list[row] : row;
         // Home-chunk slots (< C) are loaded before the header arrives: they
@@ -1091,6 +1091,253 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
             }
             __syncthreads();
         }
+    }
+}
+
+// Reads lane `l`'s copy of a 64-bit value (l uniform).
+__device__ __forceinline__ int64_t readlane64(int64_t x, int l) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)x >> 32), l);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Rows of 3D lists whose elements reserve SEG slots at a time (Hex20: one
+// reservation of 20 per element and row), so entry i lies in segment i / SEG
+// and a segment is one element's row strip with SEG distinct columns.  One
+// wave per row, keys and values loaded into registers, then:
+//   1. LDS hash set of the columns -> unique index q (ballot order);
+//   2. segments ranked by element id;
+//   3. FP64 accumulators acc[q][9] in LDS, zeroed, then the segments added
+//      one at a time in element-id order: inside a step the columns are
+//      distinct, so each accumulator sees 0 + v1 + v2 + ... in element order,
+//      the serial order of every other path (same bits);
+//   4. column ranks by counting (16 columns per LDS round trip);
+//   5. a lane per column writes its 3 x 3 block into the 3 DOF rows.
+// Loads: a row's keys, then its values, then the next row's header, all
+// issued together; the header and the row starts are spread over lanes
+// (read back with readlane where used) so that no load is waited for before
+// its value is needed: a uniform load would be moved to a scalar register at
+// once, waiting for every load issued before it.
+// Rows outside that shape (a length that is not a multiple of SEG, a segment
+// with mixed element ids or a repeated column, two segments of one element,
+// more than UMAX columns) are listed in `defer` (count misc[MISC_DEFER]) for
+// k_rows_merge_reg.
+constexpr int SEG_UMAX = 96;
+template <typename VT, typename RP, int EPL, int SEG>
+__global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* row_ptr, int32_t* col_ind,
+                                                          double* val, int32_t* defer) {
+    constexpr int B = 9, D = 3;
+    constexpr int CAP = 64 * EPL;
+    constexpr int NS = (CAP + SEG - 1) / SEG;
+    constexpr int TS = 2 * CAP <= 128 ? 128 : 2 * CAP <= 256 ? 256 : 512;
+    constexpr int UMAX = SEG_UMAX;
+    static_assert(NS <= 32, "segment masks are 32-bit");
+    static_assert(TS * sizeof(int32_t) <= UMAX * B * sizeof(double), "the hash table lives in the accumulators");
+    // The accumulators hold the hash table until every entry has its q.
+    __shared__ __align__(16) double acc[UMAX * B];
+    __shared__ __align__(16) int32_t ucol[UMAX + 4];
+    __shared__ int32_t rq[UMAX];
+    __shared__ uint32_t smask[UMAX];
+    __shared__ int32_t seid[NS];
+    __shared__ int32_t srank[NS];
+    int32_t* table = reinterpret_cast<int32_t*>(acc);
+    const int lane = threadIdx.x;
+    constexpr int shift = 32 - (31 - __builtin_clz(TS));
+    int64_t a = xcd_block(blockIdx.x, gridDim.x);
+    // Header of the next row: lane l holds word l % ALLS_DIR.
+    int32_t hv = p.hdr[min(a, p.n_nodes - 1) * ALLS_DIR + (lane & (ALLS_DIR - 1))];
+    for (; a < p.n_nodes; a += gridDim.x) {
+        const int32_t hd = hv;   // this row's header
+        const int32_t n = min(__builtin_amdgcn_readlane(hd, 0), ALLS_DIR * p.C);
+        const int64_t an = a + gridDim.x;
+        if (n > CAP) {   // the host picks EPL from the count pass; never taken
+            if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], n);
+            hv = p.hdr[min(an, p.n_nodes - 1) * ALLS_DIR + (lane & (ALLS_DIR - 1))];
+            continue;
+        }
+        // Unconditional loads (lanes past the row read its slot 0, ignored):
+        // a load under a branch makes the compiler wait for every load at
+        // the first use of any, where a straight run lets the keys be used
+        // while the values are still in flight.
+        int2 kk[EPL];
+        VT vv[EPL][B];
+        const int lim = min(n, p.C);
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            kk[k] = p.keys[a * p.C + (i < lim ? i : 0)];
+        }
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + (i < lim ? i : 0)) * B;
+#pragma unroll
+            for (int b = 0; b < B; ++b) vv[k][b] = src[b];
+        }
+        // Starts of the 3 DOF rows (lane dr holds row dr's).
+        const int64_t rv = (int64_t)row_ptr[a * D + min(lane, D - 1)];
+        hv = p.hdr[min(an, p.n_nodes - 1) * ALLS_DIR + (lane & (ALLS_DIR - 1))];
+        if (n > p.C) {   // overflow chunks (rows longer than the chunk)
+            RowDir dir;
+#pragma unroll
+            for (int j = 0; j < ALLS_DIR; ++j) dir.d[j] = __builtin_amdgcn_readlane(hd, j);
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) {
+                const int i = lane + 64 * k;
+                if (i >= p.C && i < n) {
+                    const int64_t ei = dir.entry(p, a, i);
+                    kk[k] = p.keys[ei];
+                    const VT* src = static_cast<const VT*>(p.vals) + ei * B;
+#pragma unroll
+                    for (int b = 0; b < B; ++b) vv[k][b] = src[b];
+                }
+            }
+            // Settle this rare path's loads here: otherwise the compiler's
+            // wait for the keys, merged over both paths, would cover the
+            // common path's value loads too.
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        const int ns = n / SEG;
+        bool ok = ns * SEG == n;
+        for (int i = lane; i < TS; i += 64) table[i] = -1;
+        for (int i = lane; i < UMAX; i += 64) smask[i] = 0u;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            if (i < n && i % SEG == 0) seid[i / SEG] = kk[k].y;
+        }
+        __syncthreads();
+        // 1. Hash set of the columns.
+        uint32_t slot[EPL];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            slot[k] = 0;
+            if (i < n) {
+                uint32_t h = col_hash(kk[k].x, shift);
+                for (int probe = 0; probe < TS; ++probe) {
+                    const int32_t old = atomicCAS(&table[h], -1, kk[k].x);
+                    if (old == -1 || old == kk[k].x) break;
+                    h = (h + 1) & (uint32_t)(TS - 1);
+                }
+                slot[k] = h;
+            }
+        }
+        __syncthreads();
+        // Unique index: occupied slots numbered in (j, lane) order.
+        int32_t u = 0;
+#pragma unroll
+        for (int j = 0; j < TS / 64; ++j) {
+            const int32_t c = table[j * 64 + lane];
+            const uint64_t m = __ballot(c >= 0);
+            const int32_t q = u + (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            if (c >= 0) {
+                if (q < UMAX) ucol[q] = c;
+                table[j * 64 + lane] = q;
+            }
+            u += __popcll(m);
+        }
+        ok = ok && u <= UMAX;
+        __syncthreads();
+        // Each entry's q; element ids constant inside a segment; no column
+        // twice in one segment.
+        int32_t eq[EPL];
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            eq[k] = 0;
+            if (i < n) {
+                const int32_t q = table[slot[k]];
+                eq[k] = q;
+                const int sg = i / SEG;
+                if (sg < ns) {
+                    bad |= seid[sg] != kk[k].y;
+                    if (q < UMAX) bad |= (atomicOr(&smask[q], 1u << sg) >> sg) & 1u;
+                }
+            }
+        }
+        // 2. Segment ranks by element id (distinct ids required).
+        const int32_t my_eid = lane < ns ? seid[lane] : 0;
+        int32_t rk = 0, same = 0;
+        for (int sg = 0; sg < ns; ++sg) {
+            const int32_t e = __builtin_amdgcn_readlane(my_eid, sg);
+            rk += e < my_eid ? 1 : 0;
+            same += e == my_eid ? 1 : 0;
+        }
+        bad |= lane < ns && same != 1;
+        ok = ok && __ballot(bad) == 0;
+        if (!ok) {
+            if (lane == 0) defer[atomicAdd(&p.misc[MISC_DEFER], 1)] = (int32_t)a;
+            __syncthreads();
+            continue;
+        }
+        if (lane < ns) srank[lane] = rk;
+        const int u4 = (u + 3) & ~3;
+        if (lane < u4 - u) ucol[u + lane] = 0x7fffffff;
+        __syncthreads();   // table reads done: the accumulators take its place
+        for (int x = lane; x < u * B; x += 64) acc[x] = 0.0;
+        int32_t erank[EPL];
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+            const int i = lane + 64 * k;
+            erank[k] = i < n ? srank[i / SEG] : -1;
+        }
+        __syncthreads();
+        // 3. Segments in element-id order.
+        for (int sr = 0; sr < ns; ++sr) {
+#pragma unroll
+            for (int k = 0; k < EPL; ++k) {
+                if (erank[k] == sr) {
+                    double* d = acc + eq[k] * B;
+#pragma unroll
+                    for (int b = 0; b < B; ++b) d[b] += (double)vv[k][b];
+                }
+            }
+        }
+        // 4. Column ranks: lanes take q = lane and lane + 64.
+        {
+            const int32_t c0 = lane < u ? ucol[lane] : 0x7fffffff;
+            const int32_t c1 = lane + 64 < u ? ucol[lane + 64] : 0x7fffffff;
+            int32_t r0 = 0, r1 = 0;
+            for (int k = 0; k < u4; k += 16) {
+                int4 w[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    w[t] = k + 4 * t < u4 ? *reinterpret_cast<const int4*>(ucol + k + 4 * t)
+                                          : make_int4(0x7fffffff, 0x7fffffff, 0x7fffffff, 0x7fffffff);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    r0 += (w[t].x < c0) + (w[t].y < c0) + (w[t].z < c0) + (w[t].w < c0);
+                    r1 += (w[t].x < c1) + (w[t].y < c1) + (w[t].z < c1) + (w[t].w < c1);
+                }
+            }
+            if (lane < u) rq[r0] = lane;
+            if (lane + 64 < u) rq[r1] = lane + 64;
+        }
+        __syncthreads();
+        // 5. A lane per column: its 3 x 3 block into the 3 DOF rows.
+        int64_t rbase[D];
+#pragma unroll
+        for (int dr = 0; dr < D; ++dr) rbase[dr] = readlane64(rv, dr);
+        for (int r = lane; r < u; r += 64) {
+            const int32_t q = rq[r];
+            const int32_t col = ucol[q];
+            double w[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) w[b] = acc[q * B + b];
+#pragma unroll
+            for (int dr = 0; dr < D; ++dr) {
+                const int64_t rs = rbase[dr] + (int64_t)r * D;
+#pragma unroll
+                for (int dc = 0; dc < D; ++dc) {
+                    col_ind[rs + dc] = col * D + dc;
+                    val[rs + dc] = w[dr * D + dc];
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1303,18 +1550,36 @@ long_rows:
         // Entries per lane: the fewer, the less LDS and registers per wave
         // (Hex20 rows of at most 160 entries take 3: 4 -> 5 waves/SIMD).
         const int epl = (max_big + 63) / 64;
+        if constexpr (B == 9 && WRITE) {
+            if (G == 0 && p.npe == 20 && epl >= 2) {
+                // Hex20 rows: segment merge, then the rows it defers (usually
+                // none) by the generic kernel over the deferred list.
+                if (epl == 2)
+                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 2, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                                       rp, ci, v, bl);
+                else if (epl == 3)
+                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 3, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                                       rp, ci, v, bl);
+                else
+                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 4, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                                       rp, ci, v, bl);
+                hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4>), dim3(1024), dim3(64), 0, s, p, bl, rp, ci, v,
+                                   MISC_DEFER);
+                return;
+            }
+        }
         if (epl == 1)
             hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
-                               rp, ci, v);
+                               rp, ci, v, MISC_BIG_COUNT);
         else if (epl == 2)
             hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 2>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
-                               rp, ci, v);
+                               rp, ci, v, MISC_BIG_COUNT);
         else if (epl == 3)
             hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 3>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
-                               rp, ci, v);
+                               rp, ci, v, MISC_BIG_COUNT);
         else
             hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
-                               rp, ci, v);
+                               rp, ci, v, MISC_BIG_COUNT);
         return;
     }
     if (cap <= MERGE_CAP_MAX) {
@@ -1396,20 +1661,20 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
                            static_cast<int32_t*>(row_ptr));
 }
 
-void launch_write(const ListPool& p, int G, int f32vals, int dpn, int32_t max_big, const int32_t* big_list,
+void launch_write(const ListPool& p, int G, int f32vals, int dpn, int32_t max_big, int32_t* big_list,
                   const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s) {
     (void)dpn;
     const int32_t mb = max_big;
     if (i64) {
         const int64_t* rp = static_cast<const int64_t*>(row_ptr);
-        if (p.B == 4) rows_dispatch<4, double, true, int64_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 2, s);
-        else if (f32vals) rows_dispatch<9, float, true, int64_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
-        else rows_dispatch<9, double, true, int64_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
+        if (p.B == 4) rows_dispatch<4, double, true, int64_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 2, s);
+        else if (f32vals) rows_dispatch<9, float, true, int64_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
+        else rows_dispatch<9, double, true, int64_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
     } else {
         const int32_t* rp = static_cast<const int32_t*>(row_ptr);
-        if (p.B == 4) rows_dispatch<4, double, true, int32_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 2, s);
-        else if (f32vals) rows_dispatch<9, float, true, int32_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
-        else rows_dispatch<9, double, true, int32_t>(p, G, mb, big_list, nullptr, nullptr, rp, col_ind, val, 3, s);
+        if (p.B == 4) rows_dispatch<4, double, true, int32_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 2, s);
+        else if (f32vals) rows_dispatch<9, float, true, int32_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
+        else rows_dispatch<9, double, true, int32_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
     }
 }
 

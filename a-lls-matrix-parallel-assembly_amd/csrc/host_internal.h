@@ -88,6 +88,11 @@ struct Workspace {
 };
 
 alls_status classify_mesh(const alls_mesh* m, const alls_opts& o, MeshInfo* mi, std::string* err);
+// List chunk size (entries) used when the caller leaves chunk_slots at 0.
+int default_chunk(int kind);
+// Chunks of the list pool: one home chunk per node plus the overflow
+// reserve (full: every chunk the entries could need).
+int64_t pool_chunks(int64_t n_nodes, int64_t entries_bound, int C, int kind, bool full);
 // Device view of the mesh; host arrays are copied into the workspace.
 alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, DevMesh* dm, hipStream_t s,
                        std::string* err);

@@ -74,9 +74,8 @@ int64_t estimate_footprint(const alls_mesh* m, const MeshInfo& mi, bool with_pat
     const int64_t B = mi.dpn * mi.dpn;
     const int64_t vsz = mi.f32 ? 4 : 8;
     const int64_t ent = mi.entries_bound;
-    const int64_t C = mi.kind == 3 ? 80 : mi.kind == 0 ? 16 : 24;
-    const int64_t bound = ent / C + 1;
-    const int64_t chunks = m->n_nodes + std::min(bound, bound / 4 + m->n_nodes / 4 + 1024);
+    const int64_t C = default_chunk(mi.kind);
+    const int64_t chunks = pool_chunks(m->n_nodes, ent, (int)C, mi.kind, false);
     const int64_t n_rows = m->n_nodes * mi.dpn;
     const int64_t nnz_bound = ent * B;
     int64_t scratch = chunks * C * (8 + 2 + B * vsz) + m->n_nodes * (32 + 8) + 1024;   // keys, group ids, values

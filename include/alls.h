@@ -111,7 +111,8 @@ typedef struct {
     double ms_insert;         /* element integration + list insertion */
     double ms_count;          /* per-row sort + unique count */
     double ms_scan;           /* device prefix scan -> row_ptr */
-    double ms_write;          /* per-row sort + merge -> col_ind, val */
+    double ms_write;          /* per-row sort + merge -> col_ind, val (with the row_ptr apply
+                                 when the pipeline ran as a captured graph: no stage events) */
     int64_t n_elems;
     int64_t n_rows;
     int64_t nnz;
