@@ -1104,44 +1104,50 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int l) {
 // Rows of 3D lists whose elements reserve SEG slots at a time (Hex20: one
 // reservation of 20 per element and row), so entry i lies in segment i / SEG
 // and a segment is one element's row strip with SEG distinct columns.  One
-// wave per row, keys and values loaded into registers, then:
+// wave per row; entries sit SPS = 64 / SEG whole segments to a register slot
+// (entry i = 60 k + lane for SEG = 20, lanes 60..63 idle), K slots, so a
+// segment lies in one slot k = s / SPS.  Then:
 //   1. LDS hash set of the columns -> unique index q (ballot order);
 //   2. segments ranked by element id;
 //   3. FP64 accumulators acc[q][9] in LDS, zeroed, then the segments added
-//      one at a time in element-id order: inside a step the columns are
-//      distinct, so each accumulator sees 0 + v1 + v2 + ... in element order,
-//      the serial order of every other path (same bits);
+//      one at a time in element-id order (one register slot, one lane group
+//      per step): inside a step the columns are distinct, so each
+//      accumulator sees 0 + v1 + v2 + ... in element order, the serial
+//      order of every other path (same bits);
 //   4. column ranks by counting (16 columns per LDS round trip);
 //   5. a lane per column writes its 3 x 3 block into the 3 DOF rows.
 // Loads: a row's keys, then its values, then the next row's header, all
-// issued together; the header and the row starts are spread over lanes
-// (read back with readlane where used) so that no load is waited for before
-// its value is needed: a uniform load would be moved to a scalar register at
-// once, waiting for every load issued before it.
-// Rows outside that shape (a length that is not a multiple of SEG, a segment
-// with mixed element ids or a repeated column, two segments of one element,
-// more than UMAX columns) are listed in `defer` (count misc[MISC_DEFER]) for
-// k_rows_merge_reg.
+// issued together and unconditionally (lanes past the row read its slot 0),
+// with the header and the row starts spread over lanes and read back with
+// readlane where used.  A load under a branch, or a uniform load (moved to a
+// scalar register at once), would make the compiler wait for every load
+// issued before it.
+// Rows outside that shape (a length that is not a multiple of SEG or above
+// the slots, a segment with mixed element ids or a repeated column, two
+// segments of one element, more than UMAX columns) are listed in `defer`
+// (count misc[MISC_DEFER]) for k_rows_merge_reg.
 constexpr int SEG_UMAX = 96;
-template <typename VT, typename RP, int EPL, int SEG>
+template <typename VT, typename RP, int K, int SEG>
 __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* row_ptr, int32_t* col_ind,
                                                           double* val, int32_t* defer) {
     constexpr int B = 9, D = 3;
-    constexpr int CAP = 64 * EPL;
-    constexpr int NS = (CAP + SEG - 1) / SEG;
-    constexpr int TS = 2 * CAP <= 128 ? 128 : 2 * CAP <= 256 ? 256 : 512;
+    constexpr int SPS = 64 / SEG;        // segments per register slot
+    constexpr int LPS = SPS * SEG;       // lanes per slot
+    constexpr int CAP = LPS * K;
+    constexpr int NS = SPS * K;
+    constexpr int TS = 256;              // hash slots (a row of more than UMAX columns is deferred)
     constexpr int UMAX = SEG_UMAX;
-    static_assert(NS <= 32, "segment masks are 32-bit");
+    static_assert(NS <= 32 && UMAX % 16 == 0 && UMAX <= 128, "segment masks, rank padding, two columns per lane");
     static_assert(TS * sizeof(int32_t) <= UMAX * B * sizeof(double), "the hash table lives in the accumulators");
     // The accumulators hold the hash table until every entry has its q.
     __shared__ __align__(16) double acc[UMAX * B];
-    __shared__ __align__(16) int32_t ucol[UMAX + 4];
+    __shared__ __align__(16) int32_t ucol[UMAX];
     __shared__ int32_t rq[UMAX];
     __shared__ uint32_t smask[UMAX];
     __shared__ int32_t seid[NS];
-    __shared__ int32_t srank[NS];
     int32_t* table = reinterpret_cast<int32_t*>(acc);
     const int lane = threadIdx.x;
+    const int sl = lane / SEG;           // segment of this lane inside a slot (SPS for idle lanes)
     constexpr int shift = 32 - (31 - __builtin_clz(TS));
     int64_t a = xcd_block(blockIdx.x, gridDim.x);
     // Header of the next row: lane l holds word l % ALLS_DIR.
@@ -1150,41 +1156,32 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
         const int32_t hd = hv;   // this row's header
         const int32_t n = min(__builtin_amdgcn_readlane(hd, 0), ALLS_DIR * p.C);
         const int64_t an = a + gridDim.x;
-        if (n > CAP) {   // the host picks EPL from the count pass; never taken
-            if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], n);
-            hv = p.hdr[min(an, p.n_nodes - 1) * ALLS_DIR + (lane & (ALLS_DIR - 1))];
-            continue;
-        }
-        // Unconditional loads (lanes past the row read its slot 0, ignored):
-        // a load under a branch makes the compiler wait for every load at
-        // the first use of any, where a straight run lets the keys be used
-        // while the values are still in flight.
-        int2 kk[EPL];
-        VT vv[EPL][B];
         const int lim = min(n, p.C);
+        int2 kk[K];
+        VT vv[K][B];
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-            const int i = lane + 64 * k;
-            kk[k] = p.keys[a * p.C + (i < lim ? i : 0)];
+        for (int k = 0; k < K; ++k) {
+            const int i = LPS * k + lane;
+            kk[k] = p.keys[a * p.C + (lane < LPS && i < lim ? i : 0)];
         }
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-            const int i = lane + 64 * k;
-            const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + (i < lim ? i : 0)) * B;
+        for (int k = 0; k < K; ++k) {
+            const int i = LPS * k + lane;
+            const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + (lane < LPS && i < lim ? i : 0)) * B;
 #pragma unroll
             for (int b = 0; b < B; ++b) vv[k][b] = src[b];
         }
         // Starts of the 3 DOF rows (lane dr holds row dr's).
         const int64_t rv = (int64_t)row_ptr[a * D + min(lane, D - 1)];
         hv = p.hdr[min(an, p.n_nodes - 1) * ALLS_DIR + (lane & (ALLS_DIR - 1))];
-        if (n > p.C) {   // overflow chunks (rows longer than the chunk)
+        if (n > p.C && n <= CAP) {   // overflow chunks (rows longer than the chunk)
             RowDir dir;
 #pragma unroll
             for (int j = 0; j < ALLS_DIR; ++j) dir.d[j] = __builtin_amdgcn_readlane(hd, j);
 #pragma unroll
-            for (int k = 0; k < EPL; ++k) {
-                const int i = lane + 64 * k;
-                if (i >= p.C && i < n) {
+            for (int k = 0; k < K; ++k) {
+                const int i = LPS * k + lane;
+                if (lane < LPS && i >= p.C && i < n) {
                     const int64_t ei = dir.entry(p, a, i);
                     kk[k] = p.keys[ei];
                     const VT* src = static_cast<const VT*>(p.vals) + ei * B;
@@ -1197,23 +1194,23 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
             // common path's value loads too.
             __builtin_amdgcn_s_waitcnt(0);
         }
-        const int ns = n / SEG;
+        const int ns = n <= CAP ? n / SEG : 0;   // segments (rows over CAP: none, deferred)
         bool ok = ns * SEG == n;
-        for (int i = lane; i < TS; i += 64) table[i] = -1;
+        reinterpret_cast<int4*>(table)[lane] = make_int4(-1, -1, -1, -1);   // TS = 256
         for (int i = lane; i < UMAX; i += 64) smask[i] = 0u;
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-            const int i = lane + 64 * k;
-            if (i < n && i % SEG == 0) seid[i / SEG] = kk[k].y;
+        for (int k = 0; k < K; ++k) {
+            const int i = LPS * k + lane;
+            if (lane < LPS && i < n && lane % SEG == 0) seid[SPS * k + sl] = kk[k].y;
         }
         __syncthreads();
         // 1. Hash set of the columns.
-        uint32_t slot[EPL];
+        uint32_t slot[K];
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-            const int i = lane + 64 * k;
+        for (int k = 0; k < K; ++k) {
+            const int i = LPS * k + lane;
             slot[k] = 0;
-            if (i < n) {
+            if (ok && lane < LPS && i < n) {
                 uint32_t h = col_hash(kk[k].x, shift);
                 for (int probe = 0; probe < TS; ++probe) {
                     const int32_t old = atomicCAS(&table[h], -1, kk[k].x);
@@ -1242,23 +1239,22 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
         __syncthreads();
         // Each entry's q; element ids constant inside a segment; no column
         // twice in one segment.
-        int32_t eq[EPL];
+        int32_t eq[K];
         bool bad = false;
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-            const int i = lane + 64 * k;
+        for (int k = 0; k < K; ++k) {
+            const int i = LPS * k + lane;
             eq[k] = 0;
-            if (i < n) {
+            if (ok && lane < LPS && i < n) {
                 const int32_t q = table[slot[k]];
                 eq[k] = q;
-                const int sg = i / SEG;
-                if (sg < ns) {
-                    bad |= seid[sg] != kk[k].y;
-                    if (q < UMAX) bad |= (atomicOr(&smask[q], 1u << sg) >> sg) & 1u;
-                }
+                const int sg = SPS * k + sl;
+                bad |= seid[sg] != kk[k].y;
+                if (q < UMAX) bad |= (atomicOr(&smask[q], 1u << sg) >> sg) & 1u;
             }
         }
-        // 2. Segment ranks by element id (distinct ids required).
+        // 2. Segment ranks by element id (distinct ids required); lane r of
+        // `order` holds the segment of rank r.
         const int32_t my_eid = lane < ns ? seid[lane] : 0;
         int32_t rk = 0, same = 0;
         for (int sg = 0; sg < ns; ++sg) {
@@ -1273,26 +1269,25 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
             __syncthreads();
             continue;
         }
-        if (lane < ns) srank[lane] = rk;
-        const int u4 = (u + 3) & ~3;
-        if (lane < u4 - u) ucol[u + lane] = 0x7fffffff;
+        if (lane < ns) smask[rk] = (uint32_t)lane;   // segment masks are done with
+        const int u16 = (u + 15) & ~15;
+        if (lane < u16 - u) ucol[u + lane] = 0x7fffffff;
         __syncthreads();   // table reads done: the accumulators take its place
-        for (int x = lane; x < u * B; x += 64) acc[x] = 0.0;
-        int32_t erank[EPL];
-#pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-            const int i = lane + 64 * k;
-            erank[k] = i < n ? srank[i / SEG] : -1;
-        }
+        for (int x = lane; 2 * x < u * B; x += 64) reinterpret_cast<double2*>(acc)[x] = make_double2(0.0, 0.0);
+        const int32_t order = lane < ns ? (int32_t)smask[lane] : 0;
         __syncthreads();
-        // 3. Segments in element-id order.
+        // 3. Segments in element-id order: one slot and lane group per step.
         for (int sr = 0; sr < ns; ++sr) {
+            const int sg = __builtin_amdgcn_readlane(order, sr);
+            const int k = sg / SPS;
+            if (sl == sg - SPS * k) {
 #pragma unroll
-            for (int k = 0; k < EPL; ++k) {
-                if (erank[k] == sr) {
-                    double* d = acc + eq[k] * B;
+                for (int kk2 = 0; kk2 < K; ++kk2) {
+                    if (kk2 == k) {   // uniform
+                        double* d = acc + eq[kk2] * B;
 #pragma unroll
-                    for (int b = 0; b < B; ++b) d[b] += (double)vv[k][b];
+                        for (int b = 0; b < B; ++b) d[b] += (double)vv[kk2][b];
+                    }
                 }
             }
         }
@@ -1301,12 +1296,10 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
             const int32_t c0 = lane < u ? ucol[lane] : 0x7fffffff;
             const int32_t c1 = lane + 64 < u ? ucol[lane + 64] : 0x7fffffff;
             int32_t r0 = 0, r1 = 0;
-            for (int k = 0; k < u4; k += 16) {
+            for (int k = 0; k < u16; k += 16) {
                 int4 w[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t)
-                    w[t] = k + 4 * t < u4 ? *reinterpret_cast<const int4*>(ucol + k + 4 * t)
-                                          : make_int4(0x7fffffff, 0x7fffffff, 0x7fffffff, 0x7fffffff);
+                for (int t = 0; t < 4; ++t) w[t] = *reinterpret_cast<const int4*>(ucol + k + 4 * t);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     r0 += (w[t].x < c0) + (w[t].y < c0) + (w[t].z < c0) + (w[t].w < c0);
@@ -1551,13 +1544,14 @@ long_rows:
         // (Hex20 rows of at most 160 entries take 3: 4 -> 5 waves/SIMD).
         const int epl = (max_big + 63) / 64;
         if constexpr (B == 9 && WRITE) {
-            if (G == 0 && p.npe == 20 && epl >= 2) {
+            const int ks = (max_big + 59) / 60;   // register slots of 60 entries
+            if (G == 0 && p.npe == 20 && ks >= 2 && ks <= 4) {
                 // Hex20 rows: segment merge, then the rows it defers (usually
                 // none) by the generic kernel over the deferred list.
-                if (epl == 2)
+                if (ks == 2)
                     hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 2, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
                                        rp, ci, v, bl);
-                else if (epl == 3)
+                else if (ks == 3)
                     hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 3, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
                                        rp, ci, v, bl);
                 else
