@@ -91,6 +91,36 @@ def test_hex20_rows_beyond_the_short_count_table(ctx):
     compare_csr(out.download(), oracle.assemble(m))
 
 
+def test_hex20_rows_of_four_register_slots(ctx):
+    # Two of the eight elements around the centre vertex of a 2x2x2 brick
+    # listed twice (as new elements): that row gets 200 entries, so the
+    # segment merge runs with four 60-entry register slots.
+    m = alls.generate(alls.GEN_HEX20, 2, 2, 2, jitter=0.1, seed=11, E_sigma=0.5)
+    c = m.conn.reshape(m.n_elems, 20)
+    conn = np.concatenate([c, c[[0, 5]]])
+    E = None if m.E_per_elem is None else np.concatenate([m.E_per_elem, m.E_per_elem[[0, 5]]])
+    m = alls.MeshArrays(m.coords, conn, 3, 20, m.uniform_type, None, m.E, E, m.nu, m.thickness)
+    out = ctx.assemble(m)
+    compare_csr(out.download(), oracle.assemble(m))
+
+
+def test_hex20_row_with_more_columns_than_the_segment_merge_holds(ctx):
+    # Six spread-out elements away from a corner node take it in place of
+    # one of their own nodes: its row gets 113 columns (over 96), so the
+    # segment merge leaves it to the generic kernel, and the rows it was
+    # added to keep their shape.
+    m = alls.generate(alls.GEN_HEX20, 3, 3, 3, jitter=0.1, seed=12)
+    conn = m.conn.reshape(m.n_elems, 20).copy()
+    x = int(conn[0, 0])
+    far = [e for e in range(m.n_elems) if x not in conn[e]][2::4][:6]
+    conn[far, 7] = x
+    m = alls.MeshArrays(m.coords, conn, 3, 20, m.uniform_type, None, m.E, None, m.nu, m.thickness)
+    out = ctx.assemble(m)
+    rp, ci, _ = out.download()
+    assert (rp[3 * x + 1] - rp[3 * x]) // 3 > 96
+    compare_csr((rp, ci, _), oracle.assemble(m))
+
+
 def test_repeat_runs_are_bitwise_identical(ctx):
     m = alls.generate(alls.GEN_T3, 80, 80, jitter=0.3, seed=3, random_diagonal=1, permute_nodes=1)
     outs = []
