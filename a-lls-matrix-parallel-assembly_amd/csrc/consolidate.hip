@@ -1396,6 +1396,107 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
     if (lane == 0 && longest > ld_agent(&p.misc[MISC_MAX_BIG])) atomicMax(&p.misc[MISC_MAX_BIG], longest);
 }
 
+// Unique-column count when every row is long (3D lists), rows of up to
+// 64 * K entries (the longer ones are listed in `defer`, count in
+// MISC_BIG_COUNT, for k_rows_hash_count with a full table).  One wave per
+// row, software-pipelined over its rows: while row a is hashed, the keys of
+// row a + g (g = grid size) and the header of row a + 2g are in flight, so
+// the memory round trip overlaps the LDS work.  Headers are spread over
+// lanes (lane l holds word l % ALLS_DIR: a uniform load would be moved to a
+// scalar register and waited for at once), every load is issued
+// unconditionally (lanes past a row read its slot 0) and every lane stores
+// the row's count, so that the number of memory operations between a load
+// and its use is fixed and the compiler waits for exactly what is used.
+template <int K>
+__global__ __launch_bounds__(64) void k_rows_count_long(ListPool p, int32_t* u, int32_t* defer) {
+    constexpr int CAP = 64 * K;
+    constexpr int TS = 2 * CAP;
+    static_assert((TS & (TS - 1)) == 0 && TS % 256 == 0, "table size: a power of two, 4 ints per lane per step");
+    __shared__ __align__(16) int32_t table[TS];
+    const int lane = threadIdx.x;
+    constexpr int shift = 32 - (31 - __builtin_clz(TS));
+    const int64_t g = gridDim.x;
+    const int64_t last = p.n_nodes - 1;
+    const int hl = lane & (ALLS_DIR - 1);
+    int32_t longest = 0;
+    int64_t a = xcd_block(blockIdx.x, gridDim.x);
+    int32_t ha = p.hdr[min(a, last) * ALLS_DIR + hl];       // header of row a
+    int32_t hb = p.hdr[min(a + g, last) * ALLS_DIR + hl];   // header of row a + g
+    int32_t kc[K];
+    {
+        const int lim = min(min(__builtin_amdgcn_readlane(ha, 0), ALLS_DIR * p.C), p.C);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = lane + 64 * k;
+            kc[k] = p.keys[min(a, last) * p.C + (i < lim ? i : 0)].x;
+        }
+        // Settle the prologue: the loop's waits are then those of its own
+        // iterations (the compiler merges the entry and back-edge states).
+        __builtin_amdgcn_s_waitcnt(0);
+    }
+    for (; a < p.n_nodes; a += g) {
+        const int32_t n = min(__builtin_amdgcn_readlane(ha, 0), ALLS_DIR * p.C);
+        const int32_t hd = ha;
+        // Loads for the rows ahead.
+        const int32_t hc = p.hdr[min(a + 2 * g, last) * ALLS_DIR + hl];
+        int32_t kn[K];
+        {
+            const int64_t b = min(a + g, last);
+            const int lim = min(min(__builtin_amdgcn_readlane(hb, 0), ALLS_DIR * p.C), p.C);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int i = lane + 64 * k;
+                kn[k] = p.keys[b * p.C + (i < lim ? i : 0)].x;
+            }
+        }
+        longest = max(longest, n);
+        const bool fits = n <= CAP;
+        if (!fits && lane == 0) defer[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
+        if (fits && n > p.C) {   // overflow chunks
+            RowDir dir;
+#pragma unroll
+            for (int j = 0; j < ALLS_DIR; ++j) dir.d[j] = __builtin_amdgcn_readlane(hd, j);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int i = lane + 64 * k;
+                if (i >= p.C && i < n) kc[k] = p.keys[dir.entry(p, a, i)].x;
+            }
+            // Settle this rare path's loads here, so that the common path's
+            // waits are not merged with it.
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+#pragma unroll
+        for (int j = 0; j < TS / 256; ++j) reinterpret_cast<int4*>(table)[j * 64 + lane] = make_int4(-1, -1, -1, -1);
+        __syncthreads();
+        int32_t mine = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = lane + 64 * k;
+            if (fits && i < n) {
+                uint32_t slot = col_hash(kc[k], shift);
+                for (int probe = 0; probe < TS; ++probe) {
+                    const int32_t old = atomicCAS(&table[slot], -1, kc[k]);
+                    if (old == -1) {
+                        ++mine;
+                        break;
+                    }
+                    if (old == kc[k]) break;
+                    slot = (slot + 1) & (uint32_t)(TS - 1);
+                }
+            }
+        }
+        int32_t total;
+        wave_excl_scan(mine, &total);
+        u[a] = total;   // a deferred row's count is rewritten by the second launch
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) kc[k] = kn[k];
+        ha = hb;
+        hb = hc;
+    }
+    if (lane == 0 && longest > ld_agent(&p.misc[MISC_MAX_BIG])) atomicMax(&p.misc[MISC_MAX_BIG], longest);
+}
+
 // ------------------------------------------------------------- scan ---
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER = 16;
@@ -1526,11 +1627,10 @@ long_rows:
                                p, G > 0 ? big_list : nullptr, u, ts, nullptr);
             return;
         }
-        // Every row (Hex20): a 512-slot table for rows of up to 256 entries
-        // (2 KB of LDS instead of 8 KB at C = 80: 5 -> 8 waves/SIMD), then
-        // the rows beyond it, listed in big_list, with the full table.
-        hipLaunchKernelGGL(k_rows_hash_count, dim3((unsigned)grid), dim3(64), 512 * sizeof(int32_t), s, p,
-                           nullptr, u, 512, bl);
+        // Every row (Hex20): rows of up to 256 entries with a 512-slot table
+        // (2 KB of LDS), then the rows beyond it, listed in big_list, with
+        // the full table.
+        hipLaunchKernelGGL(k_rows_count_long<4>, dim3((unsigned)grid), dim3(64), 0, s, p, u, bl);
         hipLaunchKernelGGL(k_rows_hash_count, dim3(2048), dim3(64), (size_t)ts * sizeof(int32_t), s, p, bl, u, ts,
                            nullptr);
         return;
