@@ -127,75 +127,37 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
 
     const int lane = threadIdx.x;
     const int64_t count = EMIT == 0 ? m.n_elems : sc.count;
-    // FP64 lists: a software pipeline over the wave's elements.  The next
-    // element's nodes are loaded at the start of this one, its coordinates
-    // and list reservations just before this element's list stores, and all
-    // of them are held in registers (q*), so the next element starts without
-    // waiting for these stores to drain: the wait counter counts stores and
-    // loads in issue order, and the fixed number of stores in between lets
-    // the compiler wait for the loads alone.  Reservations are taken a
-    // whole element ahead; an element with a bad node makes the whole call
-    // fail, so slots reserved for it are never read.
-    constexpr bool pipe = EMIT == 0 && sizeof(T) == 8;
-    const int64_t g = gridDim.x;
-    int32_t qn = 0, qpos = 0;
-    double qx = 0.0, qy = 0.0, qz = 0.0;
-    auto fetch_rest = [&](bool more) {   // coordinates and reservations for node qn
-        const bool ok = qn >= 0 && qn < m.n_nodes;
-        const int64_t nc = ok ? qn : 0;
-        qx = m.coords[3 * nc + 0];
-        qy = m.coords[3 * nc + 1];
-        qz = m.coords[3 * nc + 2];
-        if (more && ok && lane < 20) qpos = list_reserve_issue(p, qn, 20);
-    };
-    int64_t idx = xcd_block(blockIdx.x, gridDim.x);
-    if (pipe) {
-        qn = m.conn[min(idx, count - 1) * m.npe + lane % 20];
-        fetch_rest(idx < count);
-        __builtin_amdgcn_s_waitcnt(0);   // the loop's waits are then those of its own iterations
-    }
     // Grid-stride over the elements (a grid sized to the device, or one block per element).
-    for (; idx < count; idx += g) {
+    for (int64_t idx = xcd_block(blockIdx.x, gridDim.x); idx < count; idx += gridDim.x) {
         const int64_t e = (EMIT != 0 && sc.elist) ? (int64_t)sc.elist[idx] : idx;
         if (lane == 0) bad = 0;
         __syncthreads();
-        // FP64 lists take the reservations from the pipeline above.  FP32
-        // reserves after the contraction: a register held across it cost
-        // 3.2 -> 3.6 ms at 80^3.
+        // FP64: the reservation atomics go out with the coordinate loads, so
+        // their round trip overlaps the integration (80^3: 8.1 -> 7.8 ms; an
+        // element with a bad node makes the whole call fail, so its reserved
+        // slots are never read).  FP32 reserves after the contraction: the
+        // register held across it cost 3.2 -> 3.6 ms there.
+        constexpr bool early = EMIT == 0 && sizeof(T) == 8;
         int32_t pos = 0;
-        if (pipe) {
-            if (lane < 20) {
-                if (qn < 0 || qn >= m.n_nodes) {
-                    bad = 1;
-                    nodes[lane] = 0;
-                } else {
-                    nodes[lane] = qn;
-                    xyz[lane * 3 + 0] = qx;
-                    xyz[lane * 3 + 1] = qy;
-                    xyz[lane * 3 + 2] = qz;
-                    pos = qpos;
-                }
-            }
-            qn = m.conn[min(idx + g, count - 1) * m.npe + lane % 20];   // the next element's nodes
-        } else if (lane < 20) {
+        if (lane < 20) {
             const int32_t n = m.conn[e * m.npe + lane];
             if (n < 0 || n >= m.n_nodes) {
                 bad = 1;
                 nodes[lane] = 0;
             } else {
                 nodes[lane] = n;
-                xyz[lane * 3 + 0] = m.coords[3 * (int64_t)n + 0];
-                xyz[lane * 3 + 1] = m.coords[3 * (int64_t)n + 1];
-                xyz[lane * 3 + 2] = m.coords[3 * (int64_t)n + 2];
+                const double cx = m.coords[3 * (int64_t)n + 0];
+                const double cy = m.coords[3 * (int64_t)n + 1];
+                const double cz = m.coords[3 * (int64_t)n + 2];
+                if (early) pos = list_reserve_issue(p, n, 20);
+                xyz[lane * 3 + 0] = cx;
+                xyz[lane * 3 + 1] = cy;
+                xyz[lane * 3 + 2] = cz;
             }
         }
         __syncthreads();
         if (bad || (m.types && m.types[e] != 2)) {
             if (lane == 0) st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
-            if (pipe) {
-                fetch_rest(idx + g < count);
-                __builtin_amdgcn_s_waitcnt(0);
-            }
             __syncthreads();
             continue;
         }
@@ -296,7 +258,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         if (EMIT == 0) {
             // Reserve 20 slots in each of the element's 20 node rows.
             if (lane < 20) {
-                if (!pipe) pos = list_reserve_issue(p, nodes[lane], 20);
+                if (!early) pos = list_reserve_issue(p, nodes[lane], 20);
                 list_note_long(p, pos, 20);
                 list_own_chunks(p, nodes[lane], pos, 20);
                 posv[lane] = pos;
@@ -312,38 +274,6 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
             }
         }
         __syncthreads();
-
-        if constexpr (pipe) {
-            fetch_rest(idx + g < count);
-            bool fast = true;
-#pragma unroll
-            for (int I = 0; I < 20; ++I) fast = fast && rbase[I] >= 0;
-            if (fast) {
-                // A fixed number of stores: lanes past block 399 repeat it
-                // (same values, same addresses).
-#pragma unroll
-                for (int k = 0; k < 7; ++k) {
-                    const int pr = min(lane + 64 * k, 399);
-                    const int I = pr / 20, Jn = pr - 20 * I;
-                    T mb[3][3];
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) mb[r][c] = M[(3 * I + r) * MS + 3 * Jn + c];
-                    const T tr = mb[0][0] + mb[1][1] + mb[2][2];
-                    const int64_t id = rbase[I] + Jn;
-                    T* dst = static_cast<T*>(p.vals) + id * 9;
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            dst[r * 3 + c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
-                    p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
-                }
-                __syncthreads();   // LDS is refilled by the next element
-                continue;
-            }
-        }
 
         for (int pr = lane; pr < 400; pr += 64) {
             const int I = pr / 20, Jn = pr % 20;
@@ -402,7 +332,6 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                     }
             }
         }
-        if (pipe) __builtin_amdgcn_s_waitcnt(0);   // the slow path's stores are settled here
         __syncthreads();   // LDS is refilled by the next element
     }
 }
