@@ -119,6 +119,27 @@ def measured_traffic(stage):
         return None, None
 
 
+def copy_rate(nbytes=1 << 30, reps=5):
+    """Device-to-device hipMemcpy of `nbytes` (the runtime's copy kernel),
+    bytes read + written per second: the HBM rate a plain streaming kernel
+    reaches on this device, beside the 8 TB/s peak."""
+    import alls
+    h = alls.hip()
+    src, dst = h.malloc(nbytes), h.malloc(nbytes)
+    try:
+        h.dtod(dst, src, nbytes)
+        h.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            h.dtod(dst, src, nbytes)
+        h.sync()
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        h.free(src)
+        h.free(dst)
+    return 2 * nbytes / dt / 1e9
+
+
 def cpu_baseline(n):
     """Oracle (serial restatement of Acc:329-380 + sorted pattern) on one core."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -399,6 +420,7 @@ def main():
     value = mesh.n_elems * a.steps / dt
     dom, roof, kb, times = roofline(mesh, st, 4 * 8, 1)
     roof["traffic"], traffic_src = measured_traffic(dom)
+    copy_gbs = copy_rate()
     pb = path_bytes(mesh, st["nnz"])
     log(f"{value:.3e} elements/s, {ms:.3f} ms/step, stages {times}")
     out.free()
@@ -414,6 +436,9 @@ def main():
                    "n_elems": mesh.n_elems, "n_dofs": mesh.n_dofs, "nnz": st["nnz"]},
         "roofline": dict(roof, kernel=dom,
                          algorithmic_bytes_per_launch=kb[dom],
+                         copy_GBs=round(copy_gbs, 1),
+                         copy_note="1 GiB device-to-device hipMemcpy on the same device, read + write bytes "
+                                   "per second: what a plain streaming kernel reaches here",
                          traffic_source=traffic_src,
                          note="algorithmic bytes: mesh reads + list entries (insert); group ids, "
                               "connectivity and values read and CSR written (write)"),
