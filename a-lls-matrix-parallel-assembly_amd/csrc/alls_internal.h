@@ -12,14 +12,16 @@
 // from a bump counter above n_nodes.  Rows are node rows: one entry carries the
 // dpn x dpn block that an element adds to DOF rows a*dpn.. and DOF columns b*dpn..
 //
-// In a Q4 or T3 mesh an element reserves ng = nn slots (4 or 3) in each of
-// its node rows, and entry q of the group is its block with column node
-// conn[e][q]: the group's slots need no key of their own, only the group's
-// element id.  Chunks hold whole groups (C is a multiple of ng).  Keys for
-// the rows that the wave-per-row kernels take are written out by the count
-// pass (keys[] is kept for that).  A mixed mesh's groups would differ in size
-// (padding its triangles to 4 slots measured 25% slower), so its entries keep
-// per-entry keys.
+// In a Q4 mesh an element reserves ng = 4 slots in each of its node rows,
+// and entry q of the group is its block with column node conn[e][q]: the
+// group's slots need no key of their own, only the group's element id.
+// Chunks hold whole groups (C is a multiple of 4).  Keys for the rows that
+// the wave-per-row kernels take are written out by the count pass (keys[] is
+// kept for that).  T3, mixed and Hex20 entries keep per-entry keys (T3
+// groups of 3 measured slower: their readers load conn[e] once per group,
+// scattered when node ids are permuted; a mixed mesh's groups would differ
+// in size, and padding its triangles to 4 slots measured 25% slower).  The
+// group code takes ng = 3 too.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -80,7 +82,7 @@ struct ListPool {
     int32_t B;                 // values per entry
     uint32_t cmagic;           // chunk_magic(C): s / C as a multiply-high
     int32_t long_row;          // reservations ending past this many entries record MISC_MAX_ROW
-    int32_t ng;                // Q4 / T3: slots per group (4 or 3); 0: per-entry keys
+    int32_t ng;                // slots per group (4 for Q4; the group code also takes 3); 0: per-entry keys
     int32_t gpc;               // 2D: groups per chunk (C / ng)
     int32_t npe;               // 2D: conn stride
     int32_t _pad;

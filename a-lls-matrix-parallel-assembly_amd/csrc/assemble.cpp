@@ -628,10 +628,13 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     const int64_t n_nodes = dm.n_nodes;
     pl.cap_chunks = pool_chunks(n_nodes, mi.entries_bound, pl.C, mi.kind, pl.full_pool);
     HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
-    // Q4 / T3 lists carry one element id per group of ng slots; the
-    // per-entry keys are written only for the rows the wave-per-row kernels
-    // take.  Mixed and Hex20 lists key every entry.
-    const int ng = mi.kind == 0 ? 4 : mi.kind == 1 ? 3 : 0;
+    // Q4 lists carry one element id per group of 4 slots; the per-entry
+    // keys are written only for the rows the wave-per-row kernels take.
+    // T3, mixed and Hex20 lists key every entry (T3 with group ids paid a
+    // scattered connectivity load per group in the fused pass, 1.34 ms
+    // against 1.04 ms keyed at 1500x1500 with permuted node ids, for 0.19 ms
+    // more insertion).
+    const int ng = mi.kind == 0 ? 4 : 0;
     HIP_TRY(ws.keys.ensure((size_t)pl.cap_chunks * pl.C * sizeof(int2)), "alloc list keys");
     if (ng) HIP_TRY(ws.geid.ensure((size_t)pl.cap_chunks * (pl.C / ng) * sizeof(int32_t)), "alloc list groups");
     HIP_TRY(ws.vals.ensure((size_t)pl.cap_chunks * pl.C * B * vsz), "alloc list values");
@@ -897,8 +900,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     Pipeline pl;
     pl.C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
     pl.C = std::min(std::max(pl.C, 4), ALLS_MAX_CHUNK);
-    if (mi.kind == 1) pl.C = std::max(3, pl.C / 3 * 3);   // whole slot groups per chunk
-    else if (mi.kind == 0) pl.C = std::max(4, pl.C / 4 * 4);
+    if (mi.kind == 0) pl.C = std::max(4, pl.C / 4 * 4);   // whole slot groups per chunk
     pl.G = small_group(mi.kind);
     pl.ntiles = scan_tile_count(n_nodes);
     // Graph replays do not record the per-stage events; the whole front and

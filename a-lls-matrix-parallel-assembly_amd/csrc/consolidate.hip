@@ -1597,11 +1597,9 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
         if constexpr (B == 4) {   // 2D: Q4 groups of 4, T3 groups of 3, mixed keys
             if (G == 16) {
                 if (p.ng == 4) ALLS_SMALL(16, 4);
-                else if (p.ng == 3) ALLS_SMALL(16, 3);
                 else ALLS_SMALL(16, 0);
             } else if (G == 32) {
                 if (p.ng == 4) ALLS_SMALL(32, 4);
-                else if (p.ng == 3) ALLS_SMALL(32, 3);
                 else ALLS_SMALL(32, 0);
             } else {
                 ALLS_SMALL(64, 0);
@@ -1719,7 +1717,6 @@ void launch_rows_fused(const ListPool& p, int G, int64_t* tstat, void* row_ptr, 
 #define ALLS_FU_NG(GG, RP)                     \
     do {                                       \
         if (p.ng == 4) ALLS_FU(GG, 4, RP);     \
-        else if (p.ng == 3) ALLS_FU(GG, 3, RP); \
         else ALLS_FU(GG, 0, RP);               \
     } while (0)
     if (G == 16) {

@@ -97,8 +97,8 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                         st.v[lane][2 * j + 1] = make_double2(k[2], k[3]);
                     }
                 }
-                if (MODE == 2) list_store_keys(p, el.nd[i], pos[i], nn, el.nd, (int32_t)e);
-                else list_store_group(p, el.nd[i], pos[i], (int32_t)e);
+                if (MODE == 0) list_store_group(p, el.nd[i], pos[i], (int32_t)e);
+                else list_store_keys(p, el.nd[i], pos[i], nn, el.nd, (int32_t)e);
             }
             __syncthreads();
 #pragma unroll
