@@ -94,6 +94,11 @@ P = C.POINTER
 vp = C.c_void_p
 
 # name -> (restype, argtypes) for every function declared in include/alls.h.
+class Peaks(C.Structure):
+    _fields_ = [("copy_GBs", C.c_double), ("mfma_f64_TFLOPs", C.c_double), ("mfma_f32_TFLOPs", C.c_double),
+                ("atomic_f64_Gops", C.c_double)]
+
+
 SIGNATURES = {
     "alls_status_str": (C.c_char_p, [C.c_int]),
     "alls_ctx_create": (C.c_int, [P(vp), C.c_int, vp]),
@@ -116,6 +121,7 @@ SIGNATURES = {
     "alls_assemble_colored": (C.c_int, [vp, vp, vp, vp, C.c_int32, P(Csr)]),
     "alls_assemble_batch": (C.c_int, [vp, C.c_int32, P(Mesh), P(Opts), P(Csr), P(C.c_int64), P(C.c_int64)]),
     "alls_autotune": (C.c_int, [vp, P(Mesh), P(Opts), P(Opts)]),
+    "alls_device_peaks": (C.c_int, [vp, P(Peaks)]),
     "alls_csr_write_mtx": (C.c_int, [vp, P(Csr), C.c_char_p]),
     "alls_mesh_save": (C.c_int, [P(Mesh), C.c_char_p]),
     "alls_mesh_load": (C.c_int, [C.c_char_p, P(HostMesh)]),
@@ -320,6 +326,12 @@ class Context:
                                         row_off.ctypes.data_as(P(C.c_int64)), nnz_off.ctypes.data_as(P(C.c_int64))),
               self)
         return out, row_off, nnz_off
+
+    def device_peaks(self):
+        """alls_device_peaks: measured copy, MFMA and FP64-atomic rates of this device."""
+        pk = Peaks()
+        check(lib().alls_device_peaks(self.handle, C.byref(pk)), self)
+        return {k: getattr(pk, k) for k, _ in Peaks._fields_}
 
     def autotune(self, mesh, precision=PREC_F64, c_mesh=None):
         """alls_autotune: the fastest launch setting for this mesh's kind and size class."""

@@ -119,27 +119,6 @@ def measured_traffic(stage):
         return None, None
 
 
-def copy_rate(nbytes=1 << 30, reps=5):
-    """Device-to-device hipMemcpy of `nbytes` (the runtime's copy kernel),
-    bytes read + written per second: the HBM rate a plain streaming kernel
-    reaches on this device, beside the 8 TB/s peak."""
-    import alls
-    h = alls.hip()
-    src, dst = h.malloc(nbytes), h.malloc(nbytes)
-    try:
-        h.dtod(dst, src, nbytes)
-        h.sync()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            h.dtod(dst, src, nbytes)
-        h.sync()
-        dt = (time.perf_counter() - t0) / reps
-    finally:
-        h.free(src)
-        h.free(dst)
-    return 2 * nbytes / dt / 1e9
-
-
 def cpu_baseline(n):
     """Oracle (serial restatement of Acc:329-380 + sorted pattern) on one core."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -420,7 +399,7 @@ def main():
     value = mesh.n_elems * a.steps / dt
     dom, roof, kb, times = roofline(mesh, st, 4 * 8, 1)
     roof["traffic"], traffic_src = measured_traffic(dom)
-    copy_gbs = copy_rate()
+    peaks = ctx.device_peaks()
     pb = path_bytes(mesh, st["nnz"])
     log(f"{value:.3e} elements/s, {ms:.3f} ms/step, stages {times}")
     out.free()
@@ -436,13 +415,15 @@ def main():
                    "n_elems": mesh.n_elems, "n_dofs": mesh.n_dofs, "nnz": st["nnz"]},
         "roofline": dict(roof, kernel=dom,
                          algorithmic_bytes_per_launch=kb[dom],
-                         copy_GBs=round(copy_gbs, 1),
-                         copy_note="1 GiB device-to-device hipMemcpy on the same device, read + write bytes "
-                                   "per second: what a plain streaming kernel reaches here",
+                         measured_copy_GBs=round(peaks["copy_GBs"], 1),
+                         frac_of_measured_copy=round(roof["achieved"] / peaks["copy_GBs"], 4),
+                         measured_note="alls_device_peaks on the same device: a 1 GiB 16-byte streaming copy, "
+                                       "bytes read + written per second",
                          traffic_source=traffic_src,
                          note="algorithmic bytes: mesh reads + list entries (insert); group ids, "
                               "connectivity and values read and CSR written (write)"),
         "stages_ms": {k: round(v, 4) for k, v in times.items()},
+        "device_peaks": {k: round(v, 2) for k, v in peaks.items()},
         "path": {"compulsory_bytes": pb, "achieved_GBs": round(pb / (ms * 1e-3) / 1e9, 1),
                  "frac_of_hbm_peak": round(pb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "host_input": {"ms_per_step": round(dt_h * 1e3, 4), "elements_per_s": round(mesh.n_elems / dt_h, 1),

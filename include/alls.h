@@ -318,6 +318,20 @@ alls_status alls_csr_write_mtx(alls_ctx* ctx, const alls_csr* csr, const char* p
 alls_status alls_mesh_save(const alls_mesh* mesh, const char* path);
 alls_status alls_mesh_load(const char* path, alls_host_mesh* out);
 
+/* ----------------------------------------------------- device rates --- */
+/* Rates measured on the context's device, the measured side of the
+ * roofline figures (the vendor side: 8 TB/s, 78.6 / 157.3 TFLOP/s).  No
+ * reference counterpart: the reference quotes no peaks.  Takes about a
+ * second and 2 GiB of device memory, freed on return. */
+typedef struct {
+    double copy_GBs;          /* 1 GiB 16-byte streaming copy, bytes read + written per second */
+    double mfma_f64_TFLOPs;   /* v_mfma_f64_16x16x4f64 on register operands, every CU */
+    double mfma_f32_TFLOPs;   /* v_mfma_f32_16x16x4f32, the same way */
+    double atomic_f64_Gops;   /* FP64 atomic adds (no return) over 16M distinct doubles */
+} alls_peaks;
+
+alls_status alls_device_peaks(alls_ctx* ctx, alls_peaks* out);
+
 #ifdef __cplusplus
 }
 #endif

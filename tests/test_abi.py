@@ -41,7 +41,8 @@ def test_struct_layouts_match_the_c_header(tmp_path):
     # with the ctypes mirrors.
     structs = {"alls_mesh": alls.Mesh, "alls_opts": alls.Opts, "alls_csr": alls.Csr,
                "alls_stats": alls.Stats, "alls_gen_spec": alls.GenSpec, "alls_host_mesh": alls.HostMesh,
-               "alls_server_config": alls.ServerConfig, "alls_request_stats": alls.RequestStats}
+               "alls_server_config": alls.ServerConfig, "alls_request_stats": alls.RequestStats,
+               "alls_peaks": alls.Peaks}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "alls.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
