@@ -121,6 +121,24 @@ def test_hex20_row_with_more_columns_than_the_segment_merge_holds(ctx):
     compare_csr((rp, ci, _), oracle.assemble(m))
 
 
+@pytest.mark.parametrize("prec", [alls.PREC_F64, alls.PREC_F32_MFMA], ids=["f64", "f32"])
+def test_hex20_odd_chunk_size(ctx, prec):
+    # 161-slot chunks: half of the rows start their groups at odd list
+    # indices (the FP64 lane-pair stores fall back to single stores there)
+    # and a group at slot 160 spans two chunks.
+    m = alls.generate(alls.GEN_HEX20, 3, 3, 3, jitter=0.1, seed=5, E_sigma=0.5)
+    out = ctx.assemble(m, chunk_slots=161, precision=prec)
+    assert ctx.stats()["chunk_slots"] == 161
+    gpu = out.download()
+    out.free()
+    if prec == alls.PREC_F64:
+        compare_csr(gpu, oracle.assemble(m))
+    else:
+        rp, ci, v, _ = oracle.assemble(m)
+        assert np.array_equal(gpu[0], rp) and np.array_equal(gpu[1], ci)
+        assert np.linalg.norm(gpu[2] - v) / np.linalg.norm(v) <= 1e-6
+
+
 def test_repeat_runs_are_bitwise_identical(ctx):
     m = alls.generate(alls.GEN_T3, 80, 80, jitter=0.3, seed=3, random_diagonal=1, permute_nodes=1)
     outs = []
