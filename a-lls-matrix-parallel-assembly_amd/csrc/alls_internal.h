@@ -5,8 +5,8 @@
 //                              hdr[a][k] = 1 + index of overflow chunk k (0 = none), k = 1..7
 //   vals  VT    [chunks][C][B] B = dpn*dpn block of the entry (VT = double, or
 //                              float for FP32-integrated Hex20)
-//   Q4/T3: geid  int32 [chunks][C/ng]  element id of each group of ng slots
-//   mixed 2D and Hex20:
+//   Q4:    geid  int32 [chunks][C/ng]  element id of each group of ng slots
+//   T3, mixed 2D and Hex20:
 //          keys  int2  [chunks][C]     (column node b, element id e) of every entry
 // Chunk a (a < n_nodes) is node-row a's home chunk; overflow chunks are taken
 // from a bump counter above n_nodes.  Rows are node rows: one entry carries the

@@ -1594,7 +1594,7 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
         }
 #define ALLS_SMALL(GG, NGG) \
         hipLaunchKernelGGL((k_rows_small<GG, R, NGG, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn)
-        if constexpr (B == 4) {   // 2D: Q4 groups of 4, T3 groups of 3, mixed keys
+        if constexpr (B == 4) {   // 2D: Q4 groups of 4; T3 and mixed keys
             if (G == 16) {
                 if (p.ng == 4) ALLS_SMALL(16, 4);
                 else ALLS_SMALL(16, 0);
