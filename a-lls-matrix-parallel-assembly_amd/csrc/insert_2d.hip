@@ -170,6 +170,13 @@ __global__ __launch_bounds__(256) void k_scatter_2d(DevMesh m, Scatter sc, int32
 // other thread touches.  Both forms sum 0 + v1 + v2 + ... in the same order.
 constexpr int GATHER_SLOTS = 9;
 
+// Node id i of an element, i a run-time value: a select chain, so that the
+// element's arrays keep constant indices and stay in registers (an indexed
+// read would place the whole element record in scratch memory).
+__device__ __forceinline__ int32_t nd_at(const Elem2D& el, int i) {
+    return i == 0 ? el.nd[0] : i == 1 ? el.nd[1] : i == 2 ? el.nd[2] : el.nd[3];
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const int64_t* inc_off,
                                                    const int32_t* inc_list, int32_t* misc) {
@@ -193,7 +200,7 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
             // registers.
 #pragma unroll 1
             for (int i = 0; i < el.nn; ++i) {
-                if (el.nd[i] != a) continue;
+                if (nd_at(el, i) != a) continue;
                 RowMoments rm;
                 el.row_moments(i, rm);
                 const int4 mp = *reinterpret_cast<const int4*>(sc.map + e * 16 + i * 4);
@@ -233,7 +240,7 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
         el.material();
 #pragma unroll 1
         for (int i = 0; i < el.nn; ++i) {
-            if (el.nd[i] != a) continue;
+            if (nd_at(el, i) != a) continue;
             RowMoments rm;
             el.row_moments(i, rm);
             const int32_t* mp = sc.map + e * 16 + i * 4;
