@@ -1574,6 +1574,68 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int32_t* u, i
     }
 }
 
+// The same row_ptr as k_scan_apply for DPN = 2 or 3 and 16-byte aligned u
+// and row_ptr.  A thread's 16 nodes own 16*DPN consecutive row_ptr entries
+// (64*DPN or 128*DPN bytes from a 16-byte boundary): they are loaded and
+// stored as 16-byte vectors instead of one scattered 4- or 8-byte access per
+// entry, which left each store instruction on 64 separate cache lines.  The
+// thread whose run crosses n takes the per-entry path.
+template <typename RP, int DPN>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply_vec(const int32_t* u, int64_t n, const int64_t* tiles,
+                                                                 RP* row_ptr) {
+    static_assert(SCAN_PER % 4 == 0, "whole int4 loads");
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_PER;
+    const bool whole = base + SCAN_PER <= n;
+    int32_t loc[SCAN_PER];
+    if (whole) {
+        const int4* u4 = reinterpret_cast<const int4*>(u + base);
+#pragma unroll
+        for (int q = 0; q < SCAN_PER / 4; ++q) {
+            const int4 x = u4[q];
+            loc[4 * q] = x.x; loc[4 * q + 1] = x.y; loc[4 * q + 2] = x.z; loc[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_PER; ++i) loc[i] = base + i < n ? u[base + i] : 0;
+    }
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_PER; ++i) s += loc[i];
+    int64_t tot;
+    int64_t off = tiles[blockIdx.x] + block_excl_scan<int64_t>(s, &tot);
+    if (whole) {
+        RP v[SCAN_PER * DPN];
+#pragma unroll
+        for (int i = 0; i < SCAN_PER; ++i) {
+#pragma unroll
+            for (int dr = 0; dr < DPN; ++dr) v[i * DPN + dr] = (RP)(DPN * (off * DPN + (int64_t)dr * loc[i]));
+            off += loc[i];
+        }
+        RP* dst = row_ptr + base * DPN;
+        if constexpr (sizeof(RP) == 4) {
+#pragma unroll
+            for (int c = 0; c < SCAN_PER * DPN / 4; ++c)
+                reinterpret_cast<int4*>(dst)[c] = make_int4(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < SCAN_PER * DPN / 2; ++c)
+                reinterpret_cast<longlong2*>(dst)[c] = make_longlong2(v[2 * c], v[2 * c + 1]);
+        }
+        if (base + SCAN_PER == n) row_ptr[n * DPN] = (RP)(off * DPN * DPN);
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_PER; ++i) {
+            const int64_t a = base + i;
+            if (a < n) {
+#pragma unroll
+                for (int dr = 0; dr < DPN; ++dr) row_ptr[a * DPN + dr] = (RP)(DPN * (off * DPN + (int64_t)dr * loc[i]));
+                off += loc[i];
+                if (a == n - 1) row_ptr[n * DPN] = (RP)(off * DPN * DPN);
+            }
+        }
+    }
+}
+
 template <int B, typename VT, bool WRITE, typename RP>
 void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big_list, int32_t* u, int32_t* bl,
                    const RP* rp, int32_t* ci, double* v, int dpn, hipStream_t s) {
@@ -1750,6 +1812,19 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
     if (ntiles == 0) {
         // No nodes: row_ptr = {0} (a fill kernel, so that graphs may hold it).
         launch_fill_i32(static_cast<int32_t*>(row_ptr), i64 ? 2 : 1, 0, s);
+        return;
+    }
+    const bool aligned = ((reinterpret_cast<uintptr_t>(u) | reinterpret_cast<uintptr_t>(row_ptr)) & 15) == 0;
+    if (aligned && (dpn == 2 || dpn == 3)) {
+        const dim3 g((unsigned)ntiles), b(SCAN_THREADS);
+        if (i64 && dpn == 2)
+            hipLaunchKernelGGL((k_scan_apply_vec<int64_t, 2>), g, b, 0, s, u, n, tiles, static_cast<int64_t*>(row_ptr));
+        else if (i64)
+            hipLaunchKernelGGL((k_scan_apply_vec<int64_t, 3>), g, b, 0, s, u, n, tiles, static_cast<int64_t*>(row_ptr));
+        else if (dpn == 2)
+            hipLaunchKernelGGL((k_scan_apply_vec<int32_t, 2>), g, b, 0, s, u, n, tiles, static_cast<int32_t*>(row_ptr));
+        else
+            hipLaunchKernelGGL((k_scan_apply_vec<int32_t, 3>), g, b, 0, s, u, n, tiles, static_cast<int32_t*>(row_ptr));
         return;
     }
     if (i64)

@@ -238,6 +238,22 @@ def test_int64_row_ptr_option(ctx):
     compare_csr(out.download(), oracle.assemble(m))
 
 
+@pytest.mark.parametrize("i64", [0, 1], ids=["rp32", "rp64"])
+@pytest.mark.parametrize("kind,kw", [
+    (alls.GEN_Q4, dict(nx=3, ny=3)),                          # 16 nodes: one whole run ending at n
+    (alls.GEN_Q4, dict(nx=1, ny=14)),                         # 30 nodes: a whole run, then a partial one
+    (alls.GEN_Q4, dict(nx=63, ny=63)),                        # 4096 nodes: exactly one scan tile
+    (alls.GEN_Q4, dict(nx=64, ny=64, jitter=0.2, seed=5)),    # 4225 nodes: a second, partial tile
+    (alls.GEN_HEX20, dict(nx=3, ny=2, nz=2, jitter=0.1, seed=6)),
+], ids=["q4_16", "q4_30", "q4_4096", "q4_4225", "hex20_3x2x2"])
+def test_row_ptr_scan_at_run_and_tile_edges(ctx, kind, kw, i64):
+    """row_ptr is written 16 nodes per thread as 16-byte vectors; runs that
+    end exactly at n, partial runs and partial tiles, both index widths."""
+    print(f"case kind={kind} {kw} i64={i64}", flush=True)
+    m, gpu = run_case(ctx, kind, kw, row_ptr_i64=i64)
+    compare_csr(gpu, oracle.assemble(m))
+
+
 def test_output_buffer_reuse(ctx):
     big = alls.generate(alls.GEN_Q4, 40, 40, jitter=0.2, seed=5)
     small = alls.generate(alls.GEN_T3, 10, 10, jitter=0.2, seed=6, random_diagonal=1)
