@@ -38,6 +38,7 @@ enum {
     ERR_SPIN_TIMEOUT = 3,
     ERR_ROW_TOO_BIG = 4,
     ERR_LOOKBACK = 5,          // fused pass: a look-back wait hit its spin limit
+    ERR_REPEATED_NODE = 6,     // scatter map: an element lists one node twice (not an error)
     ERR_SLOTS = 8
 };
 // misc[] layout (int32 each, one buffer so a single copy brings them back).

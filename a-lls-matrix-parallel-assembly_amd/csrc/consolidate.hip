@@ -259,6 +259,10 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
     uint64_t key = KEY_INF;
     int32_t slot = lane;
     if (lane < n) key = pack_key(kh);
+    // Sorted by (column, element, slot): slots are distinct, so the order is
+    // total and the same on every run.  Equal (column, element) keys come
+    // from an element that lists a node twice; their slots follow the
+    // element's (row node, column node) order, the serial order.
 #pragma unroll
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
@@ -267,7 +271,8 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
             const int32_t os = (int32_t)xor_u32((uint32_t)slot, j, G);
             const bool up = (lane & k) == 0;
             const bool lower = (lane & j) == 0;
-            const bool take = (lower == up) ? (ok < key) : (ok > key);
+            const bool less = ok < key || (ok == key && os < slot);
+            const bool take = (lower == up) ? less : !less;
             if (take) {
                 key = ok;
                 slot = os;
@@ -679,8 +684,11 @@ __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list
                     const int l = i ^ j;
                     if (l > i) {
                         const uint64_t ka = sk[i], kb = sk[l];
+                        const uint16_t sa = ss[i], sb = ss[l];
                         const bool up = (i & k) == 0;
-                        if (up ? (ka > kb) : (ka < kb)) {
+                        // (key, slot) order: total, slots are distinct.
+                        const bool a_gt = ka > kb || (ka == kb && sa > sb);
+                        if (up ? a_gt : !a_gt) {
                             sk[i] = kb;
                             sk[l] = ka;
                             const uint16_t t = ss[i];
@@ -856,7 +864,7 @@ __global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* li
                 const int32_t x = memb[m];
                 const int32_t ex = eeid[x];
                 int k = m - 1;
-                while (k >= g0 && eeid[memb[k]] > ex) {
+                while (k >= g0 && (eeid[memb[k]] > ex || (eeid[memb[k]] == ex && memb[k] > x))) {
                     memb[k + 1] = memb[k];
                     --k;
                 }
@@ -1049,7 +1057,7 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
                 const int32_t x = memb[m];
                 const int32_t ex = eeid[x];
                 int k = m - 1;
-                while (k >= g0 && eeid[memb[k]] > ex) {
+                while (k >= g0 && (eeid[memb[k]] > ex || (eeid[memb[k]] == ex && memb[k] > x))) {
                     memb[k + 1] = memb[k];
                     --k;
                 }

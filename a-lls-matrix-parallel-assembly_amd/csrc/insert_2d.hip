@@ -168,6 +168,10 @@ __global__ __launch_bounds__(256) void k_scatter_2d(DevMesh m, Scatter sc, int32
 // slot s is updated by a predicated add, and the row is stored once at the
 // end.  Longer rows add into the CSR rows a*2 and a*2+1 in memory, which no
 // other thread touches.  Both forms sum 0 + v1 + v2 + ... in the same order.
+// An element that lists node a twice appears twice in a's incidence list
+// (once per conn slot); it is visited once, and that visit adds the blocks
+// of every local index of a in ascending order (row node, then column
+// node), the serial order.
 constexpr int GATHER_SLOTS = 9;
 
 // Node id i of an element, i a run-time value: a select chain, so that the
@@ -190,8 +194,11 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
         double acc[GATHER_SLOTS][4];
 #pragma unroll
         for (int q = 0; q < GATHER_SLOTS; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0;
+        int64_t prev = -1;
         for (int64_t k = k0; k < k1; ++k) {
             const int64_t e = inc_list[k];
+            if (e == prev) continue;   // the list is sorted: a repeat is the same element again
+            prev = e;
             Elem2D el;
             if (!el.load_geom<MODE>(m, e, misc)) continue;
             el.material();
@@ -233,8 +240,11 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
     }
     const int64_t r2 = rp_at(sc.row_ptr, sc.i64, 2 * a + 2);
     for (int64_t q = r0; q < r2; ++q) sc.val[q] = 0.0;
+    int64_t prev = -1;
     for (int64_t k = k0; k < k1; ++k) {
         const int64_t e = inc_list[k];
+        if (e == prev) continue;
+        prev = e;
         Elem2D el;
         if (!el.load_geom<MODE>(m, e, misc)) continue;
         el.material();
@@ -244,6 +254,27 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
             RowMoments rm;
             el.row_moments(i, rm);
             const int32_t* mp = sc.map + e * 16 + i * 4;
+            bool twice = false;   // a column node listed twice by the element
+#pragma unroll
+            for (int j = 1; j < 4; ++j)
+#pragma unroll
+                for (int j2 = 0; j2 < j; ++j2) twice |= j < el.nn && mp[j] == mp[j2];
+            if (twice) {
+                // Coincident columns: load-add-store one block after the
+                // other, so the second add sees the first.
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (j >= el.nn) continue;
+                    double kb[4];
+                    el.row_block(rm, j, kb);
+                    double* v = sc.val + r0 + 2 * (int64_t)mp[j];
+                    const double2 lo = *reinterpret_cast<const double2*>(v);
+                    const double2 hi = *reinterpret_cast<const double2*>(v + rowlen);
+                    *reinterpret_cast<double2*>(v) = make_double2(lo.x + kb[0], lo.y + kb[1]);
+                    *reinterpret_cast<double2*>(v + rowlen) = make_double2(hi.x + kb[2], hi.y + kb[3]);
+                }
+                continue;
+            }
             // All loads of the element's blocks first, then all stores: the
             // element's columns are distinct, and issuing the loads together
             // keeps them in flight instead of chaining load-add-store.
@@ -311,6 +342,14 @@ __global__ __launch_bounds__(256) void k_map(DevMesh m, const RP* rp, const int3
             }
         }
         if (!open) break;
+    }
+    if (i == 0) {   // an element that lists one node twice (the value-only Hex20 gather needs to know)
+        bool twice = false;
+#pragma unroll
+        for (int j = 1; j < NN; ++j)
+#pragma unroll
+            for (int k = 0; k < j; ++k) twice |= j < nn && tgt[j] == tgt[k];
+        if (twice) st_agent(&misc[MISC_ERR + ERR_REPEATED_NODE], 1);
     }
 #pragma unroll
     for (int j = 0; j < NN; ++j) {

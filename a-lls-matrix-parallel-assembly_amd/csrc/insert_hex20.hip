@@ -90,6 +90,27 @@ __device__ __forceinline__ void hex20_dshape(int n, double xi, double eta, doubl
     }
 }
 
+// Equal row nodes among the element's 20 node lanes: rank = lanes below
+// with the same node, cnt = lanes with it, lead = the lowest of them.
+// Lanes 20..63 keep (0, 1, lane).
+__device__ __forceinline__ void hex20_match(int32_t n, int lane, int32_t* rk, int32_t* cnt, int32_t* lead) {
+    int32_t r = 0, c = 0, l = lane;
+#pragma unroll
+    for (int j = 0; j < 20; ++j) {
+        const int32_t nj = __builtin_amdgcn_readlane(n, j);
+        if (nj == n) {
+            ++c;
+            r += j < lane ? 1 : 0;
+            l = j < l ? j : l;
+        }
+    }
+    if (lane < 20) {
+        *rk = r;
+        *cnt = c;
+        *lead = l;
+    }
+}
+
 __device__ __forceinline__ f64x4 mfma(double a, double b, f64x4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -138,10 +159,17 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         // slots are never read).  FP32 reserves after the contraction: the
         // register held across it cost 3.2 -> 3.6 ms there.
         constexpr bool early = EMIT == 0 && sizeof(T) == 8;
+        const int32_t n = lane < 20 ? m.conn[e * m.npe + lane] : -1;
+        const bool valid = lane < 20 && n >= 0 && n < m.n_nodes;
+        // Lanes naming the same row node (an element that lists a node
+        // twice) share one reservation: the lowest such lane reserves 20
+        // slots for each of them and lane order assigns the strips, so a
+        // row's strips of one element lie in the element's row-node order.
+        int32_t rk = 0, cnt = 1, lead = lane;
+        if (EMIT == 0) hex20_match(n, lane, &rk, &cnt, &lead);
         int32_t pos = 0;
         if (lane < 20) {
-            const int32_t n = m.conn[e * m.npe + lane];
-            if (n < 0 || n >= m.n_nodes) {
+            if (!valid) {
                 bad = 1;
                 nodes[lane] = 0;
             } else {
@@ -149,7 +177,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                 const double cx = m.coords[3 * (int64_t)n + 0];
                 const double cy = m.coords[3 * (int64_t)n + 1];
                 const double cz = m.coords[3 * (int64_t)n + 2];
-                if (early) pos = list_reserve_issue(p, n, 20);
+                if (early && rk == 0) pos = list_reserve_issue(p, n, 20 * cnt);
                 xyz[lane * 3 + 0] = cx;
                 xyz[lane * 3 + 1] = cy;
                 xyz[lane * 3 + 2] = cz;
@@ -158,6 +186,9 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         __syncthreads();
         if (bad || (m.types && m.types[e] != 2)) {
             if (lane == 0) st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
+            // Reservations already taken still publish the chunks they own:
+            // later elements of the same rows wait for those chunks.
+            if (early && valid && rk == 0) list_own_chunks(p, n, pos, 20 * cnt);
             __syncthreads();
             continue;
         }
@@ -257,10 +288,13 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         const T mu = (T)(E / (2.0 * (1.0 + nu)));
         if (EMIT == 0) {
             // Reserve 20 slots in each of the element's 20 node rows.
+            if (lane < 20 && rk == 0) {
+                if (!early) pos = list_reserve_issue(p, nodes[lane], 20 * cnt);
+                list_note_long(p, pos, 20 * cnt);
+                list_own_chunks(p, nodes[lane], pos, 20 * cnt);
+            }
+            pos = __shfl(pos, lead, 64) + 20 * rk;   // this lane's strip of the shared reservation
             if (lane < 20) {
-                if (!early) pos = list_reserve_issue(p, nodes[lane], 20);
-                list_note_long(p, pos, 20);
-                list_own_chunks(p, nodes[lane], pos, 20);
                 posv[lane] = pos;
                 // The group's chunk, found once per row instead of once per
                 // entry (an overflow chunk's directory word is an L2 access).

@@ -33,6 +33,7 @@ struct alls_pattern {
     DevBuf inc_cnt, inc_off, inc_cursor, inc_list, tiles;   // node -> element incidence, sorted by element id
     bool have_inc = false;
     int32_t umax = 0;                                       // Hex20: most column nodes in one node row
+    bool repeated_node = false;                             // some element lists one node twice
     int32_t id = 0;                                         // stamped into outputs holding the pattern
 };
 
@@ -244,6 +245,7 @@ alls_status pattern_build(Workspace& ws, const alls_mesh* mesh, const alls_opts&
         if (err) *err = "scatter map: a column was missing from the pattern";
         return fail(ALLS_E_HIP);
     }
+    p->repeated_node = flags[MISC_ERR + ERR_REPEATED_NODE] != 0;
     if ((rc = build_incidence(p, s, err)) != ALLS_OK) return fail(rc);
     if (mi.kind == 3 && (rc = hex20_row_limit(p, ws.misc.as<int32_t>(), s, err)) != ALLS_OK) return fail(rc);
     if (hipStreamSynchronize(s) != hipSuccess) return fail(hip_fail(hipGetLastError(), "sync pattern", err));
@@ -261,7 +263,10 @@ alls_status pattern_values(Workspace& ws, alls_pattern* p, const double* E_dev, 
     DevMesh dm = p->dm;
     dm.E_elem = E_dev;
     dm.E = E_scalar;
-    if (p->mi.kind == 3 && p->umax > pair_gather_max_cols())   // rows too wide for the gather's LDS
+    // Hex20 rows too wide for the gather's LDS, or an element that lists a
+    // node twice (two of its blocks would land on one column in the same
+    // gather step): the plain pipeline, whose values are the same bits.
+    if (p->mi.kind == 3 && (p->umax > pair_gather_max_cols() || p->repeated_node))
         return run_alls(ws, dm, p->mi, p->opts, out, s, st, err);
     Scatter sc = make_scatter(p, out);
     sc.count = dm.n_elems;
@@ -488,6 +493,12 @@ alls_status alls_assemble_colored(alls_ctx* ctx, const alls_pattern* cpat, const
     if (col->n_elems != p->dm.n_elems) {
         *err = "colouring belongs to another mesh";
         return ALLS_E_INVALID;
+    }
+    if (p->mi.kind == 3 && p->repeated_node) {
+        // The Hex20 coloured kernel adds an element's 400 blocks from
+        // parallel lanes: two blocks of one element on one column would race.
+        *err = "coloured Hex20 assembly of an element that lists a node twice";
+        return ALLS_E_UNSUPPORTED;
     }
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;

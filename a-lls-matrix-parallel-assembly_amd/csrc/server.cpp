@@ -65,6 +65,30 @@ struct CacheEntry {
     uint64_t last_use = 0;
     int precision = 0;
     int64_t n_elems = 0, n_nodes = 0;
+    // What a value-only repeat takes from the cached pattern besides the
+    // geometry (which the key promises): element kind layout and material.
+    int32_t dim = 0, npe = 0, uniform_type = 0;
+    bool mixed = false;
+    double nu = 0.0, thickness = 0.0;
+
+    void describe(const alls_mesh& m, const alls_opts& o) {
+        precision = o.precision;
+        n_elems = m.n_elems;
+        n_nodes = m.n_nodes;
+        dim = m.dim;
+        npe = m.nodes_per_elem;
+        uniform_type = m.uniform_type;
+        mixed = m.elem_type != nullptr;
+        nu = m.nu;
+        thickness = m.thickness;
+    }
+    // A request may reuse this pattern: same key (checked by the caller),
+    // precision, sizes, element kinds and material constants; only E differs.
+    bool serves(const alls_mesh& m, const alls_opts& o) const {
+        return precision == o.precision && n_elems == m.n_elems && n_nodes == m.n_nodes && dim == m.dim &&
+               npe == m.nodes_per_elem && uniform_type == m.uniform_type && mixed == (m.elem_type != nullptr) &&
+               nu == m.nu && (dim == 3 || thickness == m.thickness);
+    }
 };
 
 // Upper-bound device bytes of one assembly: returns output (+ pattern) bytes;
@@ -180,9 +204,7 @@ struct alls_server {
     bool runs_plain(const Request& q) const {
         if (!q.key) return true;
         auto c = cache.find(q.key);
-        if (c != cache.end() && c->second.precision == q.opts.precision && c->second.n_elems == q.mesh.n_elems &&
-            c->second.n_nodes == q.mesh.n_nodes)
-            return false;
+        if (c != cache.end() && c->second.serves(q.mesh, q.opts)) return false;
         auto sc = seen.find(q.key);
         return building.count(q.key) || !worth_building(q, sc == seen.end() ? 0 : sc->second);
     }
@@ -242,9 +264,7 @@ struct alls_server {
                     for (int attempt = 0; attempt < 2 && !r && !queue.empty(); ++attempt) {
                         std::shared_ptr<Request> q = queue.front();
                         auto c = q->key ? cache.find(q->key) : cache.end();
-                        const bool can_hit = c != cache.end() && c->second.precision == q->opts.precision &&
-                                             c->second.n_elems == q->mesh.n_elems &&
-                                             c->second.n_nodes == q->mesh.n_nodes;
+                        const bool can_hit = c != cache.end() && c->second.serves(q->mesh, q->opts);
                         const auto sc = q->key ? seen.find(q->key) : seen.end();
                         const int prior = sc == seen.end() ? 0 : sc->second;
                         const bool can_build = q->key && !can_hit && !building.count(q->key) &&
@@ -338,9 +358,7 @@ struct alls_server {
                     e.pat = p;
                     e.bytes = pattern_bytes(p);
                     e.last_use = ++use_clock;
-                    e.precision = r->opts.precision;
-                    e.n_elems = r->mesh.n_elems;
-                    e.n_nodes = r->mesh.n_nodes;
+                    e.describe(r->mesh, r->opts);
                     if ((int)cache.size() >= std::max(1, cfg.pattern_cache_entries)) {
                         auto victim = cache.end();
                         for (auto it = cache.begin(); it != cache.end(); ++it)

@@ -41,6 +41,15 @@ def hex20_workload(n, seed=0xA11500C4):
     return alls.generate(alls.GEN_HEX20, n, n, n, jitter=0.15, seed=seed, E=1.0, nu=0.3)
 
 
+def c3_workload(name, n, seed=0xA11500C3):
+    """Unstructured T3 ("t3": every cell split along a random diagonal) or
+    mixed Q4/T3 ("mixed": 50% of cells split) on an n x n grid, jitter
+    +-0.3h, node ids randomly permuted, E=1, nu=0.3."""
+    import alls
+    kind, kw = {"t3": (alls.GEN_T3, {}), "mixed": (alls.GEN_MIXED, {"split_fraction": 0.5})}[name]
+    return alls.generate(kind, n, n, jitter=0.3, seed=seed, random_diagonal=1, permute_nodes=1, **kw)
+
+
 def time_steps(ctx, dmesh, steps, warmup, **opts):
     """W untimed + K timed assemblies; returns (seconds, mean stats, out)."""
     import alls
@@ -65,40 +74,73 @@ def time_steps(ctx, dmesh, steps, warmup, **opts):
 
 
 def path_bytes(mesh, nnz):
-    """Compulsory bytes of the whole path: inputs + output CSR."""
+    """Compulsory bytes of one assembly: the mesh read once (coordinates,
+    connectivity, per-element E) and the CSR written once (row_ptr 4 B per
+    row + 1, col_ind 4 B and val 8 B per nonzero).  The list entries that
+    A-LLS writes and reads back in between are intermediate traffic, not
+    algorithmic bytes."""
     inputs = mesh.coords.nbytes + mesh.conn.nbytes + (mesh.E_per_elem.nbytes if mesh.E_per_elem is not None else 0)
+    if mesh.elem_type is not None:
+        inputs += mesh.elem_type.nbytes
     return inputs + 4 * (mesh.n_dofs + 1) + 12 * nnz
 
 
-def kernel_bytes(mesh, st, val_bytes, key_bytes):
-    """Algorithmic bytes per launch of each pipeline stage.  Q4 lists hold
-    val_bytes per entry and one 4-byte element id per group of 4 entries
-    (key_bytes = 1 per entry); the consolidation reads each entry's column
-    from the connectivity (read once: conn bytes)."""
-    inputs = mesh.coords.nbytes + mesh.conn.nbytes + (mesh.E_per_elem.nbytes if mesh.E_per_elem is not None else 0)
-    conn = mesh.conn.nbytes
+def intermediate_bytes(mesh, st, val_bytes, key_bytes):
+    """List traffic per stage (bytes moved beyond the compulsory ones): the
+    entries the insertion appends (values + keys or group ids) and the
+    consolidation passes read back."""
     entries = st["list_entries"]
-    nnz = st["nnz"]
-    n_nodes = mesh.n_nodes
-    return {
-        # reads the mesh, appends every list entry and group id, bumps one count per row
-        "insert": inputs + entries * (val_bytes + key_bytes) + 4 * n_nodes,
-        # reads each row's count and group ids, the connectivity, writes one count per row
-        "count": entries * key_bytes + conn + 8 * n_nodes,
-        # reads group ids, connectivity and values, writes col_ind (4 B) + val (8 B) per nonzero and row_ptr
-        "write": entries * (val_bytes + key_bytes) + conn + 12 * nnz + 8 * (mesh.n_dofs + 1),
-    }
+    return {"insert": entries * (val_bytes + key_bytes),
+            "count": entries * key_bytes,
+            "write": entries * (val_bytes + key_bytes)}
 
 
-def roofline(mesh, st, val_bytes, key_bytes):
-    kb = kernel_bytes(mesh, st, val_bytes, key_bytes)
-    times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
-    dom = max(times, key=times.get)
-    achieved = kb[dom] / (times[dom] * 1e-3) / 1e9
-    return dom, {
-        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-    }, kb, times
+def hbm_roofline(comp_bytes, stage_ms, stage_kernel):
+    """Roofline of the dominant kernel: the compulsory bytes of one assembly
+    (one launch of each kernel) over that kernel's average duration (hipEvents
+    on the library's stream around the stage, which holds that one kernel)."""
+    dom = max(stage_ms, key=stage_ms.get)
+    achieved = comp_bytes / (stage_ms[dom] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "stage": dom,
+            "kernel": stage_kernel.get(dom, dom), "kernel_ms": round(stage_ms[dom], 4),
+            "algorithmic_bytes_per_launch": comp_bytes}
+
+
+def path_block(comp_bytes, ms):
+    gbs = comp_bytes / (ms * 1e-3) / 1e9
+    return {"compulsory_bytes": comp_bytes, "ms_per_step": round(ms, 4), "achieved_GBs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def reservations(mesh):
+    """Reservation atomics of one insertion: one per (element, row node)
+    before any merging of equal rows (Q4 4, T3 3, Hex20 20 per element)."""
+    if mesh.elem_type is not None:
+        nn = np.where(mesh.elem_type == 0, 4, 3)
+        return int(nn.sum())
+    return mesh.n_elems * {4: 4, 3: 3, 20: 20}[mesh.npe]
+
+
+def atomic_block(n_res, insert_ms, atomic_peak_gops, issued=None):
+    """Reservation atomics per second of the insertion kernel against the
+    device's measured rate of atomic adds to distinct addresses."""
+    rate = n_res / (insert_ms * 1e-3) / 1e9
+    out = {"reservations_per_launch": n_res, "achieved_Gops": round(rate, 2),
+           "measured_atomic_Gops": round(atomic_peak_gops, 2),
+           "frac_of_measured": round(rate / atomic_peak_gops, 4) if atomic_peak_gops else None}
+    if issued is not None:
+        out["atomics_issued_per_launch"] = issued
+    return out
+
+
+def footprint(mesh, st):
+    """HBM held by one assembly: inputs, list headers + pool, output CSR."""
+    inputs = path_bytes(mesh, 0) - 4 * (mesh.n_dofs + 1)
+    rp = (mesh.n_dofs + 1) * (8 if st["nnz"] >= 2 ** 31 else 4)
+    out = rp + 12 * st["nnz"]
+    lists = st["scratch_bytes"]
+    return {"inputs": inputs, "lists": lists, "output": out, "total": inputs + lists + out}
 
 
 # Kernel behind each pipeline stage of the Q4 workload (rocprofv3 names).
@@ -106,33 +148,50 @@ STAGE_KERNEL = {"insert": "k_insert_2d<0>", "count": "k_count_quad<4, double>",
                 "write": "k_rows_small<16, 1, 4, 4, double, true, int>"}
 
 
-def measured_traffic(stage):
-    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (same workload), already scaled by the
     copy-kernel calibration; None when the summary is absent."""
     path = os.path.join(REPO, "profiles", "alls_bench_hbm_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
-        return t["kernels"][STAGE_KERNEL[stage]]["traffic_bytes"], t["source"]
+        return t["kernels"][kernel]["traffic_bytes"], t["source"]
     except (OSError, KeyError, ValueError):
         return None, None
 
 
-def cpu_baseline(n):
-    """Oracle (serial restatement of Acc:329-380 + sorted pattern) on one core."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mesh):
+    """Oracle (serial restatement of Acc:329-380 + sorted pattern) on the
+    whole workload mesh, one thread pinned to one core."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
-    m = q4_workload(n)
-    t0 = time.perf_counter()
-    rp, ci = oracle.pattern(m)
-    t1 = time.perf_counter()
-    oracle.assemble(m, rp, ci, with_abs=False)
-    t2 = time.perf_counter()
-    return {"value": round(m.n_elems / (t2 - t0), 1), "unit": "elements/s", "cores": 1, "kind": "port",
-            "sample": f"Q4 {n}x{n} ({m.n_elems} elements) of the same generator; pattern {t1 - t0:.2f} s + "
-                      f"serial fill {t2 - t1:.2f} s",
-            "seconds": round(t2 - t0, 2)}
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        rp, ci = oracle.pattern(mesh)
+        t1 = time.perf_counter()
+        oracle.assemble(mesh, rp, ci, with_abs=False)
+        t2 = time.perf_counter()
+    finally:
+        os.sched_setaffinity(0, old)
+    return {"value": round(mesh.n_elems / (t2 - t0), 1), "unit": "elements/s", "cores": 1, "kind": "port",
+            "sample": f"the whole workload mesh ({mesh.n_elems} elements): sorted pattern {t1 - t0:.2f} s + "
+                      f"serial element-order fill {t2 - t1:.2f} s, one thread pinned to core {core}",
+            "seconds": round(t2 - t0, 2), "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
 
 def time_calls(fn, steps, warmup):
@@ -235,8 +294,8 @@ def c3_compare(ctx, n, steps):
     permuted node ids): A-LLS full path, value-only and GPU-coloured paths."""
     import alls
     res = {}
-    for name, kind, kw in (("t3", alls.GEN_T3, {}), ("mixed", alls.GEN_MIXED, {"split_fraction": 0.5})):
-        m = alls.generate(kind, n, n, jitter=0.3, seed=0xA11500C3, random_diagonal=1, permute_nodes=1, **kw)
+    for name in ("t3", "mixed"):
+        m = c3_workload(name, n)
         dm = alls.DeviceMesh(m)
         dt, st, out = time_steps(ctx, dm, steps, 1)
         out.free()

@@ -257,8 +257,13 @@ typedef struct {
 
 alls_status alls_server_create(const alls_server_config* cfg, alls_server** srv);
 /* Enqueues a request; the mesh arrays must stay valid until it completes.
- * mesh_key != 0 enables the pattern cache for this mesh.  Returns ALLS_E_BUSY
- * if the request alone exceeds the whole budget. */
+ * mesh_key != 0 enables the pattern cache for this mesh.  A key promises the
+ * same node coordinates and connectivity on every request that carries it:
+ * only E (scalar or per element) may change.  A cached pattern serves a
+ * request only if precision, node and element counts, dim, nodes_per_elem,
+ * uniform_type, the presence of elem_type, nu and (2D) thickness all match;
+ * otherwise the request runs the plain path (or rebuilds the pattern).
+ * Returns ALLS_E_BUSY if the request alone exceeds the whole budget. */
 alls_status alls_server_submit(alls_server* srv, const alls_mesh* mesh, uint64_t mesh_key,
                                const alls_opts* opts, int64_t* ticket);
 /* Blocks until the request completes.  Its CSR moves into *out (the caller

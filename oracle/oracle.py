@@ -57,6 +57,15 @@ def lib():
         L.or_color_greedy.restype = C.c_int32
         L.or_coloring_conflicts.argtypes = [P(OrMesh), vp]
         L.or_coloring_conflicts.restype = C.c_int64
+        # csr_checks.c: full-size CSR invariants (OpenMP)
+        L.ck_rows_sorted_unique.argtypes = [C.c_int64, vp, vp, C.c_int64]
+        L.ck_rows_sorted_unique.restype = C.c_int64
+        L.ck_symmetry.argtypes = [C.c_int64, vp, vp, vp, vp, vp]
+        L.ck_symmetry.restype = C.c_double
+        L.ck_matvec.argtypes = [C.c_int64, vp, vp, vp, vp, vp]
+        L.ck_matvec.restype = None
+        L.ck_frobenius.argtypes = [C.c_int64, vp, vp, vp]
+        L.ck_frobenius.restype = None
         _lib = L
     return _lib
 
@@ -174,3 +183,45 @@ def coloring_conflicts(m, colors):
     om = OracleMesh(m)
     colors = np.ascontiguousarray(colors, np.int32)
     return lib().or_coloring_conflicts(C.byref(om.s), _p(colors))
+
+
+# ----------------------------------------------- full-size CSR invariants ---
+# (csr_checks.c; rp must be int64, ci int32, v float64, all contiguous)
+
+def _csr_args(rp, ci, v=None):
+    rp = np.ascontiguousarray(rp, np.int64)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = None if v is None else np.ascontiguousarray(v, np.float64)
+    return rp, ci, v
+
+
+def rows_sorted_unique_violations(rp, ci, n_cols=None):
+    rp, ci, _ = _csr_args(rp, ci)
+    n = rp.size - 1
+    return int(lib().ck_rows_sorted_unique(n, _p(rp), _p(ci), n if n_cols is None else n_cols))
+
+
+def symmetry(rp, ci, v):
+    """(max |K_ij - K_ji|, entries without a transpose, max |K_ij|)."""
+    rp, ci, v = _csr_args(rp, ci, v)
+    miss = C.c_int64(0)
+    vmax = C.c_double(0.0)
+    d = lib().ck_symmetry(rp.size - 1, _p(rp), _p(ci), _p(v), C.addressof(miss), C.addressof(vmax))
+    return float(d), int(miss.value), float(vmax.value)
+
+
+def matvec(rp, ci, v, x):
+    rp, ci, v = _csr_args(rp, ci, v)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty(rp.size - 1)
+    lib().ck_matvec(rp.size - 1, _p(rp), _p(ci), _p(v), _p(x), _p(y))
+    return y
+
+
+def frobenius_rel(a, b):
+    """||a - b|| / ||b||."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    out = np.empty(2)
+    lib().ck_frobenius(a.size, _p(a), _p(b), _p(out))
+    return float(np.sqrt(out[0]) / max(np.sqrt(out[1]), 1e-300))

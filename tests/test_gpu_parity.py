@@ -56,27 +56,120 @@ def test_hex20_f32_mfma_frobenius(ctx):
     assert fro <= 1e-6
 
 
-@pytest.mark.parametrize("kind,kw", [
+REPEATED = [
     (alls.GEN_Q4, dict(nx=7, ny=6, jitter=0.1, seed=3)),
     (alls.GEN_T3, dict(nx=7, ny=6, jitter=0.1, seed=3, random_diagonal=1)),
     (alls.GEN_MIXED, dict(nx=7, ny=6, jitter=0.1, seed=3, random_diagonal=1)),
     (alls.GEN_HEX20, dict(nx=3, ny=2, nz=2, jitter=0.1, seed=3)),
-], ids=["q4", "t3", "mixed", "hex20"])
-def test_element_with_a_repeated_node(ctx, kind, kw):
-    # A degenerate element (one node id twice) puts two contributions of one
-    # element on the same (row, column): the list layouts must keep both (the
-    # Hex20 group merge falls back to scanning its groups for such a row).
+]
+
+
+def repeated_node_mesh(kind, kw):
+    """The generated mesh with one element's second node replaced by its
+    first.  In a mixed mesh that element is a quadrilateral: a triangle that
+    lists a node twice has zero area, and the reference's CST formula
+    (T3:177-205, no determinant clamp) gives NaN blocks for it, which every
+    path reproduces (checked by position only)."""
     m = alls.generate(kind, **kw)
     conn = m.conn.reshape(m.n_elems, m.npe).copy()
     e = m.n_elems // 2
+    if m.elem_type is not None:
+        e += int(np.argmax(m.elem_type[e:] == alls.Q4))
     conn[e, 1] = conn[e, 0]
-    m = alls.MeshArrays(m.coords, conn, m.dim, m.npe, m.uniform_type, m.elem_type, m.E, m.E_per_elem, m.nu,
-                        m.thickness)
+    return e, alls.MeshArrays(m.coords, conn, m.dim, m.npe, m.uniform_type, m.elem_type, m.E, m.E_per_elem, m.nu,
+                              m.thickness)
+
+
+@pytest.mark.parametrize("kind,kw", REPEATED, ids=["q4", "t3", "mixed", "hex20"])
+def test_element_with_a_repeated_node(ctx, kind, kw):
+    # A degenerate element (one node id twice) puts two contributions of one
+    # element on the same (row, column): the list layouts must keep both (the
+    # Hex20 group merge falls back to scanning its groups for such a row),
+    # and the two are summed in the element's (row node, column node) order,
+    # the serial order, so repeated runs give the same bits.
+    e, m = repeated_node_mesh(kind, kw)
     print(f"repeated node kind {kind} element {e}", flush=True)
     out = ctx.assemble(m)
     gpu = out.download()
-    out.free()
     compare_csr(gpu, oracle.assemble(m))
+    for _ in range(2):
+        out = ctx.assemble(m, out=out)
+        assert same_csr(out.download(), gpu)
+    out.free()
+
+
+def same_csr(a, b):
+    """Bitwise equal CSR triples (NaN values compared by position)."""
+    return (np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+            and np.array_equal(a[2], b[2], equal_nan=True))
+
+
+def repeated_node_fan():
+    """40 elements around node 0 (a 121-entry row, past the register path):
+    triangles, except element 3, the quadrilateral (0, 4, 5, 5) that lists
+    node 5 twice.  Row 0 then holds two blocks of one element on column 5,
+    and row 5 sees the element twice in its incidence list."""
+    k = 40
+    m = fan_mesh(k)
+    conn = np.full((k, 4), -1, np.int32)
+    conn[:, :3] = m.conn.reshape(k, 3)
+    conn[3] = [0, 4, 5, 5]
+    types = np.full(k, alls.T3, np.uint8)
+    types[3] = alls.Q4
+    return alls.MeshArrays(m.coords, conn, 2, 4, uniform_type=alls.Q4, elem_type=types)
+
+
+@pytest.mark.parametrize("case", ["q4", "t3", "mixed", "hex20", "fan"])
+def test_value_only_and_cache_hit_with_a_repeated_node(ctx, case):
+    # The cached-pattern path (and a server cache hit, which runs it) must
+    # give the plain assembly's bits for a mesh with a degenerate element.
+    if case == "fan":
+        m = repeated_node_fan()
+    else:
+        kind, kw = REPEATED[["q4", "t3", "mixed", "hex20"].index(case)]
+        m = repeated_node_mesh(kind, kw)[1]
+    print(f"repeated node value-only case {case}", flush=True)
+    E2 = np.exp(0.5 * np.random.default_rng(21).standard_normal(m.n_elems))
+    m2 = m.with_E(E2)
+    out = ctx.assemble(m2)
+    plain = out.download()
+    out.free()
+    compare_csr(plain, oracle.assemble(m2))
+    pat = alls.Pattern(ctx, m)
+    pat.out.free()
+    out = pat.values(E2)
+    assert same_csr(out.download(), plain)
+    out.free()
+    pat.free()
+    with alls.Server(n_streams=2, cache_policy=alls.CACHE_EAGER) as srv:
+        srv.wait(srv.submit(m, key=3), keep_output=False)
+        csr, st = srv.wait(srv.submit(m2, key=3))
+        assert st["cache_hit"] == 1
+        got = srv.download(csr)
+        srv.release(csr)
+    assert same_csr(got, plain)
+
+
+def test_hex20_invalid_node_with_overflow_chunks_fails_fast(ctx):
+    # Elements with a bad node reserve list slots before the check (FP64);
+    # with 40-slot chunks (vertex rows of 160 entries take 3 overflow chunks)
+    # some of those reservations own a chunk, which they must still publish,
+    # or the other elements of that row would wait for it until their spin
+    # limit.  The call reports the bad node at once and the context stays
+    # usable.
+    import time
+    m = alls.generate(alls.GEN_HEX20, 3, 3, 3, jitter=0.1, seed=8)
+    compare_csr(ctx.assemble(m, chunk_slots=40).download(), oracle.assemble(m))
+    conn = m.conn.reshape(m.n_elems, 20).copy()
+    conn[::3, 19] = m.n_nodes + 5
+    bad = alls.MeshArrays(m.coords, conn, 3, 20, m.uniform_type, None, m.E, None, m.nu, m.thickness)
+    t0 = time.perf_counter()
+    with pytest.raises(alls.AllsError) as ei:
+        ctx.assemble(bad, chunk_slots=40)
+    dt = time.perf_counter() - t0
+    assert ei.value.status == alls.E_INVALID
+    assert dt < 0.1, dt
+    compare_csr(ctx.assemble(m, chunk_slots=40).download(), oracle.assemble(m))
 
 
 def test_hex20_rows_beyond_the_short_count_table(ctx):

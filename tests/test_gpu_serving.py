@@ -121,3 +121,24 @@ def test_small_budget_serializes_and_refuses_oversized():
         ts = [srv.submit(small.with_E(np.full(small.n_elems, 1.0 + i))) for i in range(8)]
         stats = [srv.wait(t, keep_output=False)[1] for t in ts]
     assert all(s["status"] == 0 for s in stats)
+
+
+def test_cache_hit_requires_the_same_material_constants():
+    # A key promises the same geometry; a request that changes nu under the
+    # same key must not take the cached pattern's values (its nu is baked
+    # into the value-only path): it rebuilds or runs plain, and is correct.
+    m = alls.generate(alls.GEN_Q4, 30, 20, jitter=0.2, seed=5)
+    m_nu = alls.MeshArrays(m.coords, m.conn, 2, 4, m.uniform_type, None, m.E, None, 0.25, m.thickness)
+    with alls.Server(n_streams=2, cache_policy=alls.CACHE_EAGER) as srv:
+        srv.wait(srv.submit(m, key=5), keep_output=False)
+        csr, st = srv.wait(srv.submit(m_nu, key=5))
+        assert st["cache_hit"] != 1
+        got = srv.download(csr)
+        srv.release(csr)
+        csr, st = srv.wait(srv.submit(m_nu, key=5))   # now the entry holds nu = 0.25
+        assert st["cache_hit"] == 1
+        again = srv.download(csr)
+        srv.release(csr)
+    ref = oracle.assemble(m_nu)
+    compare_csr(got, ref)
+    compare_csr(again, ref)

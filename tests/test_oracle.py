@@ -14,8 +14,8 @@ import pytest
 
 import alls
 import oracle
-from _checks import (csr_matvec, csr_symmetry_error, element_areas_2d, plane_stress_D, rigid_modes,
-                     solid_D, uniform_strain_field)
+from _checks import (HEX_NAT, csr_matvec, csr_symmetry_error, element_areas_2d, hex20_dshape_numpy,
+                     plane_stress_D, rigid_modes, solid_D, uniform_strain_field)
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -109,37 +109,6 @@ def test_t3_cst_closed_form(seed):
 
 
 # --------------------------------------------------------------- Hex20 ---
-HEX_NAT = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1],
-                    [-1, 1, 1], [0, -1, -1], [1, 0, -1], [0, 1, -1], [-1, 0, -1], [0, -1, 1], [1, 0, 1],
-                    [0, 1, 1], [-1, 0, 1], [-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], float)
-
-
-def hex20_dshape_numpy(p):
-    """Serendipity derivatives from the product form (written independently)."""
-    xi, eta, ze = p
-    d = np.zeros((3, 20))
-    for n, (a, b, c) in enumerate(HEX_NAT):
-        if a != 0 and b != 0 and c != 0:
-            fa, fb, fc = 1 + a * xi, 1 + b * eta, 1 + c * ze
-            s = a * xi + b * eta + c * ze - 2
-            d[0, n] = 0.125 * (a * fb * fc * s + fa * fb * fc * a)
-            d[1, n] = 0.125 * (b * fa * fc * s + fa * fb * fc * b)
-            d[2, n] = 0.125 * (c * fa * fb * s + fa * fb * fc * c)
-        else:
-            # 0.25 * (1 - q^2) * (1 + r r_n) * (1 + s s_n) with q the zero coordinate
-            v = np.array([xi, eta, ze]); nat = np.array([a, b, c])
-            z = int(np.nonzero(nat == 0)[0][0])
-            f = [1 - v[z] ** 2 if k == z else 1 + nat[k] * v[k] for k in range(3)]
-            df = [-2 * v[z] if k == z else nat[k] for k in range(3)]
-            for k in range(3):
-                prod = 0.25 * df[k]
-                for l in range(3):
-                    if l != k:
-                        prod *= f[l]
-                d[k, n] = prod
-    return d
-
-
 def hex20_ke_numpy(X, E, nu, ngp):
     g, w = np.polynomial.legendre.leggauss(ngp)
     D = solid_D(E, nu)
@@ -312,3 +281,41 @@ def test_golden_c1_fixture():
     r = 2 * (32 * 65 + 32)
     diag = v[rp[r]:rp[r + 1]][ci[rp[r]:rp[r + 1]] == r][0]
     assert abs(diag - g["interior_diag"]) < 1e-14
+
+
+def test_csr_checks_catch_planted_errors():
+    # The full-size GPU tests rest on these checks: each must see a planted
+    # mistake of the kind it guards against.
+    m = alls.generate(alls.GEN_T3, 12, 9, jitter=0.2, seed=4, random_diagonal=1, permute_nodes=1, E_sigma=0.5)
+    rp, ci, v, _ = oracle.assemble(m)
+    x = np.random.default_rng(1).standard_normal(rp.size - 1)
+    assert np.allclose(oracle.matvec(rp, ci, v, x), csr_matvec(rp, ci, v, x), rtol=0, atol=1e-12)
+    assert oracle.rows_sorted_unique_violations(rp, ci) == 0
+    d, miss, vmax = oracle.symmetry(rp, ci, v)
+    assert miss == 0 and d <= 1e-14 * vmax
+    assert abs(d / vmax - csr_symmetry_error(rp, ci, v)) < 1e-15
+    bad = ci.copy()
+    bad[rp[7] + 1], bad[rp[7] + 2] = bad[rp[7] + 2], bad[rp[7] + 1]   # two columns swapped
+    assert oracle.rows_sorted_unique_violations(rp, bad) == 1
+    bad = ci.copy()
+    bad[rp[7] + 1] = bad[rp[7]]                                      # a repeated column
+    assert oracle.rows_sorted_unique_violations(rp, bad) == 1
+    w = v.copy()
+    k = next(k for k in range(rp[30], rp[31]) if ci[k] != 30)
+    w[k] += 1e-9 * vmax                                              # one off-diagonal value off
+    assert oracle.symmetry(rp, ci, w)[0] > 0.5e-9 * vmax
+    assert oracle.frobenius_rel(w, v) > 0 and oracle.frobenius_rel(v, v) == 0
+
+
+def test_hex20_gauss_volumes_tile_the_cube():
+    from _checks import hex20_gauss_volumes
+    m = alls.generate(alls.GEN_HEX20, 3, 2, 2, jitter=0.15, seed=7)
+    vol = hex20_gauss_volumes(m.coords, m.conn)
+    assert (vol > 0).all() and abs(vol.sum() - 1.0) < 1e-13
+    X = m.coords.reshape(-1, 3)
+    c = m.conn.reshape(-1, 20)
+    for e in (0, 5):
+        g, w = np.polynomial.legendre.leggauss(3)
+        ref = sum(np.linalg.det(hex20_dshape_numpy((a, b, cc)) @ X[c[e]]) * wa * wb * wc
+                  for a, wa in zip(g, w) for b, wb in zip(g, w) for cc, wc in zip(g, w))
+        assert abs(vol[e] - ref) < 1e-15
