@@ -244,7 +244,14 @@ def coloring_compare(ctx, mesh, dm, steps, alls_ms):
     return res
 
 
-def hex20_compare(ctx, n, steps):
+# MFMA work of one Hex20 element in k_hex20: the 10 upper 16x16 tiles of the
+# padded 64x64 product X^T (W X), 7 k-steps of 16x16x4 (2048 flops each).
+HEX20_MFMA_FLOPS = 10 * 7 * 2 * 16 * 16 * 4
+# The dense B^T D B contraction count per element (60 x 60 x 162 x 2).
+HEX20_DENSE_FLOPS = 1166400
+
+
+def hex20_compare(ctx, n, steps, peaks):
     """FP64 vs FP32-MFMA Hex20 assembly of an n^3 brick (one GPU)."""
     import alls
     m = hex20_workload(n)
@@ -253,9 +260,31 @@ def hex20_compare(ctx, n, steps):
     v = {}
     for name, prec in (("f64", alls.PREC_F64), ("f32_mfma", alls.PREC_F32_MFMA)):
         dt, st, out = time_steps(ctx, dm, steps, 1, precision=prec)
-        res[name] = {"ms_per_step": round(dt / steps * 1e3, 3), "elements_per_s": round(m.n_elems * steps / dt, 1),
+        ms = dt / steps * 1e3
+        res[name] = {"ms_per_step": round(ms, 3), "elements_per_s": round(m.n_elems * steps / dt, 1),
                      "ms_insert": round(st["ms_insert"], 3), "ms_count": round(st["ms_count"], 3),
                      "ms_write": round(st["ms_write"], 3), "nnz": st["nnz"]}
+        times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
+        t = "double" if name == "f64" else "float"
+        kernels = {"insert": f"k_hex20<{t}, 0>", "count": "k_rows_count_long<4>",
+                   "write": f"k_rows_merge_seg<{t}, int, 3, 20>"}
+        pb = path_bytes(m, st["nnz"])
+        res[name]["roofline"] = hbm_roofline(pb, times, kernels)
+        res[name]["path"] = path_block(pb, ms)
+        peak = FP64_PEAK_TFLOPS if name == "f64" else FP32_PEAK_TFLOPS
+        meas = peaks["mfma_f64_TFLOPs"] if name == "f64" else peaks["mfma_f32_TFLOPs"]
+        real = m.n_elems * HEX20_MFMA_FLOPS / (st["ms_insert"] * 1e-3) / 1e12
+        res[name]["mfma"] = {
+            "kernel": kernels["insert"], "kernel_ms": round(st["ms_insert"], 4),
+            "flops_per_element": HEX20_MFMA_FLOPS, "achieved_TFLOPs": round(real, 2), "peak_TFLOPs": peak,
+            "utilisation": round(real / peak, 4), "measured_peak_TFLOPs": round(meas, 2),
+            "frac_of_measured": round(real / meas, 4),
+            "dense_equivalent_TFLOPs": round(m.n_elems * HEX20_DENSE_FLOPS / (st["ms_insert"] * 1e-3) / 1e12, 1),
+            "note": "MFMA flops the kernel issues (10 tiles x 7 k-steps of 16x16x4 per element) over its time; "
+                    "dense_equivalent counts the B^T D B contraction (1,166,400 flops per element), which the "
+                    "M-form does not perform"}
+        res[name]["atomics"] = atomic_block(reservations(m), st["ms_insert"], peaks["atomic_f64_Gops"])
+        res[name]["hbm_footprint"] = footprint(m, st)
         v[name] = out.download()[2]
         dt_h, out = time_calls(lambda o: ctx.assemble(m, precision=prec, out=o), 2, 1)
         res[name]["host_input_ms"] = round(dt_h * 1e3, 3)
@@ -289,7 +318,7 @@ def hex20_compare(ctx, n, steps):
     return res
 
 
-def c3_compare(ctx, n, steps):
+def c3_compare(ctx, n, steps, peaks):
     """Unstructured T3 and mixed Q4/T3 n x n (jitter 0.3h, random diagonals,
     permuted node ids): A-LLS full path, value-only and GPU-coloured paths."""
     import alls
@@ -299,9 +328,19 @@ def c3_compare(ctx, n, steps):
         dm = alls.DeviceMesh(m)
         dt, st, out = time_steps(ctx, dm, steps, 1)
         out.free()
-        r = {"elements": m.n_elems, "nnz": st["nnz"], "ms_per_step": round(dt / steps * 1e3, 3),
+        ms = dt / steps * 1e3
+        r = {"elements": m.n_elems, "nnz": st["nnz"], "ms_per_step": round(ms, 3),
              "elements_per_s": round(m.n_elems * steps / dt, 1), "ms_insert": round(st["ms_insert"], 3),
              "ms_count": round(st["ms_count"], 3), "ms_write": round(st["ms_write"], 3), "big_rows": st["big_rows"]}
+        times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
+        mode = 1 if name == "t3" else 2
+        kernels = {"insert": f"k_insert_2d<{mode}>", "write": "k_rows_fused<32, 0, int>"}
+        pb = path_bytes(m, st["nnz"])
+        r["roofline"] = hbm_roofline(pb, times, kernels)
+        r["path"] = path_block(pb, ms)
+        r["intermediate_bytes"] = intermediate_bytes(m, st, 4 * 8, 8)
+        r["atomics"] = atomic_block(reservations(m), st["ms_insert"], peaks["atomic_f64_Gops"])
+        r["hbm_footprint"] = footprint(m, st)
         pat = alls.Pattern(ctx, m, c_mesh=dm.c_mesh())
         pat.out.free()
         dtv, o2 = time_calls(lambda o: pat.values(out=o), steps, 1)
@@ -429,7 +468,8 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=2000, help="Q4 grid size (workload n x n)")
-    ap.add_argument("--cpu-n", type=int, default=1000, help="CPU baseline sample grid size")
+    ap.add_argument("--cpu-n", type=int, default=0,
+                    help="CPU baseline on a Q4 n x n sample instead of the whole workload (0: the workload)")
     ap.add_argument("--hex-n", type=int, default=80)
     ap.add_argument("--c3-n", type=int, default=1500, help="T3 / mixed grid size")
     ap.add_argument("--no-cpu", action="store_true")
@@ -456,10 +496,11 @@ def main():
     dt, st, out = time_steps(ctx, dm, a.steps, a.warmup)
     ms = dt / a.steps * 1e3
     value = mesh.n_elems * a.steps / dt
-    dom, roof, kb, times = roofline(mesh, st, 4 * 8, 1)
-    roof["traffic"], traffic_src = measured_traffic(dom)
-    peaks = ctx.device_peaks()
+    times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
     pb = path_bytes(mesh, st["nnz"])
+    roof = hbm_roofline(pb, times, STAGE_KERNEL)
+    roof["traffic"], traffic_src = measured_traffic(roof["kernel"])
+    peaks = ctx.device_peaks()
     log(f"{value:.3e} elements/s, {ms:.3f} ms/step, stages {times}")
     out.free()
     dt_h, out = time_calls(lambda o: ctx.assemble(mesh, out=o), max(3, a.steps // 2), 1)
@@ -472,19 +513,23 @@ def main():
                                f"{mesh.n_dofs} DOF, nnz {st['nnz']}), jitter +-0.2h, E~lognormal(0,0.5), "
                                f"nu=0.3, t=1, 2x2 Gauss, FP64, inputs resident in HBM",
                    "n_elems": mesh.n_elems, "n_dofs": mesh.n_dofs, "nnz": st["nnz"]},
-        "roofline": dict(roof, kernel=dom,
-                         algorithmic_bytes_per_launch=kb[dom],
+        "roofline": dict(roof,
                          measured_copy_GBs=round(peaks["copy_GBs"], 1),
                          frac_of_measured_copy=round(roof["achieved"] / peaks["copy_GBs"], 4),
                          measured_note="alls_device_peaks on the same device: a 1 GiB 16-byte streaming copy, "
                                        "bytes read + written per second",
                          traffic_source=traffic_src,
-                         note="algorithmic bytes: mesh reads + list entries (insert); group ids, "
-                              "connectivity and values read and CSR written (write)"),
+                         note="achieved = compulsory bytes of one assembly (mesh read once + CSR written once, "
+                              f"{pb / mesh.n_elems:.1f} B/element) / the dominant kernel's average duration; "
+                              "traffic = that kernel's measured HBM bytes per launch, which include the list "
+                              "entries it reads back (intermediate_bytes)"),
         "stages_ms": {k: round(v, 4) for k, v in times.items()},
         "device_peaks": {k: round(v, 2) for k, v in peaks.items()},
-        "path": {"compulsory_bytes": pb, "achieved_GBs": round(pb / (ms * 1e-3) / 1e9, 1),
-                 "frac_of_hbm_peak": round(pb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "path": path_block(pb, ms),
+        "intermediate_bytes": intermediate_bytes(mesh, st, 4 * 8, 1),
+        "atomics": atomic_block(reservations(mesh), st["ms_insert"], peaks["atomic_f64_Gops"],
+                                st.get("atomics_issued")),
+        "hbm_footprint": footprint(mesh, st),
         "host_input": {"ms_per_step": round(dt_h * 1e3, 4), "elements_per_s": round(mesh.n_elems / dt_h, 1),
                        "note": "same workload with the mesh in host memory: pinned staging and the PCIe "
                                "copy inside the timed region (not the metric's value)"},
@@ -512,17 +557,18 @@ def main():
     if "c3" in ex:
         try:
             log(f"T3 / mixed {a.c3_n}x{a.c3_n}")
-            extra["c3"] = c3_compare(ctx, a.c3_n, 5)
+            extra["c3"] = c3_compare(ctx, a.c3_n, 5, peaks)
         except Exception as e:
             extra["c3"] = {"error": str(e)}
     if not a.no_cpu:
-        log(f"cpu baseline on Q4 {a.cpu_n}x{a.cpu_n}")
-        res["cpu_baseline"] = cpu_baseline(a.cpu_n)
+        cm = mesh if a.cpu_n in (0, a.n) else q4_workload(a.cpu_n)
+        log(f"cpu baseline on {cm.n_elems} elements")
+        res["cpu_baseline"] = cpu_baseline(cm)
         res["cpu_baseline"]["gpu_speedup"] = round(value / res["cpu_baseline"]["value"], 1)
     if "hex20" in ex:
         try:
             log(f"Hex20 {a.hex_n}^3 FP64 vs FP32-MFMA")
-            extra["hex20"] = hex20_compare(ctx, a.hex_n, 3)
+            extra["hex20"] = hex20_compare(ctx, a.hex_n, 3, peaks)
         except Exception as e:  # report, do not hide the main line
             extra["hex20"] = {"error": str(e)}
     ctx.close()   # frees the context's scratch before the server takes its HBM budget
