@@ -45,7 +45,15 @@ struct Elem2D {
     // Node ids, coordinates and material of element e, no integration.
     template <int MODE>
     __device__ __forceinline__ bool load_geom(const DevMesh& m, int64_t e, int32_t* misc) {
-#pragma clang fp contract(off)
+        if (!load_nodes<MODE>(m, e, misc)) return false;
+        load_coords(m, e);
+        return true;
+    }
+
+    // Node ids of element e (checked against the mesh); false (and an error
+    // flag) when a node id is out of range.
+    template <int MODE>
+    __device__ __forceinline__ bool load_nodes(const DevMesh& m, int64_t e, int32_t* misc) {
         const int kind = MODE == 2 ? (int)m.types[e] : MODE;
         if (kind > 1) {
             st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
@@ -69,6 +77,11 @@ struct Elem2D {
             st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
             return false;
         }
+        return true;
+    }
+
+    // Coordinates and material of element e, after load_nodes.
+    __device__ __forceinline__ void load_coords(const DevMesh& m, int64_t e) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             x[i] = y[i] = 0.0;
@@ -81,7 +94,6 @@ struct Elem2D {
         E = m.E_elem ? m.E_elem[e] : m.E;
         nu = m.nu;
         t = m.thickness;
-        return true;
     }
 
     // f = E / (1 - nu^2) and c = (1 - nu) / 2, after load_geom (kept apart
