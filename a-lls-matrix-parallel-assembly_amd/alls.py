@@ -107,6 +107,7 @@ SIGNATURES = {
     "alls_last_stats": (C.c_int, [vp, P(Stats)]),
     "alls_device_count": (C.c_int, []),
     "alls_assemble": (C.c_int, [vp, P(Mesh), P(Opts), P(Csr)]),
+    "alls_assemble_noalloc": (C.c_int, [vp, P(Mesh), P(Opts), P(Csr)]),
     "alls_csr_download": (C.c_int, [vp, P(Csr), vp, vp, vp]),
     "alls_csr_free": (None, [vp, P(Csr)]),
     "alls_pattern_create": (C.c_int, [vp, P(Mesh), P(Opts), P(vp), P(Csr)]),
@@ -309,6 +310,17 @@ class Context:
         return out
 
 
+    def assemble_noalloc(self, dmesh, out, precision=PREC_F64, chunk_slots=0, block_threads=0, grid_per_cu=0,
+                         c_mesh=None):
+        """alls_assemble_noalloc: a DeviceMesh into the caller's CallerCsr
+        buffers, with no device allocation by the library."""
+        o = Opts()
+        o.precision, o.chunk_slots, o.block_threads = precision, chunk_slots, block_threads
+        o.grid_per_cu, o.timing = grid_per_cu, 1
+        m = c_mesh if c_mesh is not None else dmesh.c_mesh()
+        check(lib().alls_assemble_noalloc(self.handle, C.byref(m), C.byref(o), C.byref(out.csr)), self)
+        return out
+
     def assemble_batch(self, meshes, precision=PREC_F64, chunk_slots=0, block_threads=0, grid_per_cu=0,
                        autotune=0, out=None, c_meshes=None):
         """alls_assemble_batch: one pipeline over all meshes (same dim).  Returns
@@ -473,6 +485,37 @@ class DeviceCsr:
     def free(self):
         if self.ctx is not None and self.ctx.handle:
             lib().alls_csr_free(self.ctx.handle, C.byref(self.csr))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class CallerCsr:
+    """Device CSR buffers owned by the caller (for alls_assemble_noalloc):
+    row_ptr of cap_rows entries (int64 if row_ptr_i64), col_ind and val of
+    cap_nnz entries, allocated here with hipMalloc and freed with hipFree."""
+
+    def __init__(self, ctx, cap_rows, cap_nnz, row_ptr_i64=0):
+        self.ctx = ctx
+        h = hip()
+        self.csr = Csr()
+        self.csr.row_ptr = h.malloc(cap_rows * (8 if row_ptr_i64 else 4))
+        self.csr.col_ind = h.malloc(max(cap_nnz, 1) * 4)
+        self.csr.val = h.malloc(max(cap_nnz, 1) * 8)
+        self.csr.cap_rows, self.csr.cap_nnz = cap_rows, cap_nnz
+        self.csr.row_ptr_is_i64 = row_ptr_i64
+
+    def download(self):
+        return DeviceCsr.download(self)
+
+    def free(self):
+        h = hip()
+        for p in (self.csr.row_ptr, self.csr.col_ind, self.csr.val):
+            h.free(p)
+        self.csr = Csr()
 
     def __del__(self):
         try:

@@ -90,6 +90,9 @@ namespace {
 std::mutex g_arena_mu;
 std::vector<Arena*> g_arenas;
 thread_local Arena* t_arena = nullptr;
+// Set for the duration of an alls_assemble_noalloc call: any device
+// allocation on this thread then fails instead of happening.
+thread_local bool t_no_alloc = false;
 
 void arena_insert_free(Arena* a, size_t off, size_t len) {
     a->free_at[off] = len;
@@ -180,7 +183,13 @@ void arena_destroy(Arena* a) {
 
 void arena_bind(Arena* a) { t_arena = a; }
 
+void no_alloc_scope(bool on) { t_no_alloc = on; }
+
 hipError_t dev_alloc(void** out, size_t bytes) {
+    if (t_no_alloc) {
+        *out = nullptr;
+        return hipErrorOutOfMemory;
+    }
     if (t_arena) {
         // Arena first; a request it cannot fit falls back to the cache below.
         void* p = arena_alloc(t_arena, std::max<size_t>(bytes, 256));
@@ -627,6 +636,31 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     const size_t vsz = mi.f32 ? sizeof(float) : sizeof(double);
     const int64_t n_nodes = dm.n_nodes;
     pl.cap_chunks = pool_chunks(n_nodes, mi.entries_bound, pl.C, mi.kind, pl.full_pool);
+    if (ws.ext) {
+        // Fixed scratch (alls_assemble_noalloc): every array is carved from
+        // it, 256-byte aligned, and nothing is allocated.
+        const int ngx = mi.kind == 0 ? 4 : 0;
+        const size_t nn1 = (size_t)std::max<int64_t>(n_nodes, 1);
+        const size_t need[] = {nn1 * ALLS_DIR * sizeof(int32_t), (size_t)pl.cap_chunks * pl.C * sizeof(int2),
+                               ngx ? (size_t)pl.cap_chunks * (pl.C / ngx) * sizeof(int32_t) : 0,
+                               (size_t)pl.cap_chunks * pl.C * B * vsz, MISC_SIZE * sizeof(int32_t),
+                               nn1 * sizeof(int32_t), nn1 * sizeof(int32_t), (size_t)(pl.ntiles + 1) * sizeof(int64_t)};
+        DevBuf* bufs[] = {&ws.hdr, &ws.keys, &ws.geid, &ws.vals, &ws.misc, &ws.u, &ws.big, &ws.tiles};
+        size_t off = 0;
+        for (size_t k = 0; k < sizeof(need) / sizeof(need[0]); ++k) off += (need[k] + 255) & ~size_t(255);
+        if (off > ws.ext_bytes) {
+            if (err)
+                *err = "the mesh needs " + std::to_string(off) + " bytes of list and consolidation scratch; "
+                       "the fixed scratch holds " + std::to_string(ws.ext_bytes);
+            return ALLS_E_UNSUPPORTED;
+        }
+        off = 0;
+        for (size_t k = 0; k < sizeof(need) / sizeof(need[0]); ++k) {
+            bufs[k]->p = need[k] ? ws.ext + off : nullptr;
+            bufs[k]->bytes = need[k];
+            off += (need[k] + 255) & ~size_t(255);
+        }
+    }
     HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
     // Q4 lists carry one element id per group of 4 slots; the per-entry
     // keys are written only for the rows the wave-per-row kernels take.
@@ -912,7 +946,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     // separate passes run on the same lists.  Measured on MI355X, the fused
     // pass saves 13% of the T3/mixed path, while for Q4 (16-lane rows, a
     // count pass of 0.32 ms at C2) the look-back costs more than it saves.
-    bool try_fused = !use_graph && pl.G == 32 && n_nodes > 0;
+    bool try_fused = !use_graph && !ws.ext && pl.G == 32 && n_nodes > 0;
     bool fused_done = false;
     for (;;) {
         alls_status rs = alloc_lists(ws, dm, mi, pl, err);
@@ -948,6 +982,16 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     alls_status rs = ALLS_OK;
     if (fused_done) {
         out->nnz = nnz;
+    } else if (ws.ext) {
+        // The caller's output buffers: checked, never reallocated.
+        out->n_rows = n_rows;
+        out->nnz = nnz;
+        if (out->cap_rows < n_rows + 1 || out->cap_nnz < nnz || (!out->row_ptr_is_i64 && nnz > (int64_t)INT32_MAX)) {
+            if (err)
+                *err = "output buffers too small: need " + std::to_string(n_rows + 1) + " row_ptr entries and " +
+                       std::to_string(nnz) + " nonzeros (nnz returned in the CSR)";
+            return ALLS_E_INVALID;
+        }
     } else {
         const int i64 = (o.row_ptr_i64 || nnz > (int64_t)INT32_MAX) ? 1 : 0;
         rs = ensure_csr(out, n_rows, nnz, i64);
@@ -1051,6 +1095,9 @@ alls_status alls_ctx_create(alls_ctx** ctx, int device, void* hip_stream) {
         delete c;
         return ALLS_E_HIP;
     }
+    // Device address of alls_assemble_noalloc's fixed scratch on this device
+    // (looked up once per device; a failure is reported by that entry point).
+    (void)noalloc_scratch_init(device);
     *ctx = c;
     return ALLS_OK;
 }
@@ -1089,6 +1136,63 @@ alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
     alls_stats st{};
     s = run_alls(ctx->ws, dm, mi, o, out, ctx->stream, &st, &ctx->err);
     if (s == ALLS_OK) ctx->stats = st;
+    return s;
+}
+
+alls_status alls_assemble_noalloc(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts, alls_csr* out) {
+    if (!ctx || !mesh || !out) return ALLS_E_INVALID;
+    alls_opts o{};
+    if (opts) o = *opts;
+    ctx->err.clear();
+    (void)hipSetDevice(ctx->device);
+    if (mesh->mem != ALLS_MEM_DEVICE) {
+        ctx->err = "alls_assemble_noalloc takes device-resident mesh arrays (host arrays need staging memory)";
+        return ALLS_E_INVALID;
+    }
+    if (o.autotune) {
+        ctx->err = "alls_assemble_noalloc does not tune (the autotuner allocates); pass explicit settings";
+        return ALLS_E_INVALID;
+    }
+    if (!out->row_ptr || out->cap_rows < 1 || (out->cap_nnz > 0 && (!out->col_ind || !out->val))) {
+        ctx->err = "alls_assemble_noalloc writes into the caller's row_ptr/col_ind/val (cap_rows, cap_nnz)";
+        return ALLS_E_INVALID;
+    }
+    char* base = static_cast<char*>(noalloc_scratch(ctx->device));
+    if (!base) {
+        ctx->err = "the fixed scratch of alls_assemble_noalloc has no device address on this device";
+        return ALLS_E_HIP;
+    }
+    MeshInfo mi;
+    alls_status s = classify_mesh(mesh, o, &mi, &ctx->err);
+    if (s != ALLS_OK) return s;
+    // A workspace view over the fixed scratch, sharing the context's pinned
+    // read-back word and events (made at context creation).
+    Workspace xs;
+    xs.ext = base;
+    xs.ext_bytes = noalloc_scratch_bytes();
+    xs.host_misc = ctx->ws.host_misc;
+    for (int i = 0; i < 8; ++i) xs.ev[i] = ctx->ws.ev[i];
+    xs.events = ctx->ws.events;
+    alls_stats st{};
+    no_alloc_scope(true);
+    DevMesh dm;
+    s = stage_mesh(xs, mesh, mi, &dm, ctx->stream, &ctx->err);
+    if (s == ALLS_OK) s = run_alls(xs, dm, mi, o, out, ctx->stream, &st, &ctx->err);
+    no_alloc_scope(false);
+    // The view owns nothing: detach it before its buffers would be released.
+    for (DevBuf* b : {&xs.hdr, &xs.keys, &xs.geid, &xs.vals, &xs.misc, &xs.u, &xs.big, &xs.tiles, &xs.coords,
+                      &xs.conn, &xs.types, &xs.E, &xs.bdesc, &xs.bsplit}) {
+        b->p = nullptr;
+        b->bytes = 0;
+    }
+    xs.host_misc = nullptr;
+    xs.events = false;
+    if (s == ALLS_OK) ctx->stats = st;
+    if (s == ALLS_E_OOM) {   // the allocation guard refused a buffer this path does not provide
+        ctx->err = "alls_assemble_noalloc: " + ctx->err + " (device inputs must be 16-byte aligned; nothing is "
+                   "allocated on this path)";
+        s = ALLS_E_UNSUPPORTED;
+    }
     return s;
 }
 

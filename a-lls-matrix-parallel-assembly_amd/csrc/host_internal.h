@@ -13,6 +13,15 @@ namespace alls {
 
 // Caching device allocator (free lists; no device-wide sync on free).
 hipError_t dev_alloc(void** out, size_t bytes);
+// While on, dev_alloc on this thread fails (hipErrorOutOfMemory) instead of
+// allocating: the alls_assemble_noalloc guarantee.
+void no_alloc_scope(bool on);
+// The fixed device scratch of alls_assemble_noalloc (noalloc.hip): its
+// address on `device`, looked up once per device (call with the device
+// current), and its size.
+hipError_t noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
 void dev_free(void* p);
 void dev_trim();
 // Device arena: one reservation carved up by the threads bound to it
@@ -81,6 +90,11 @@ struct Workspace {
     hipEvent_t ev[8] = {};
     bool events = false;
     GraphSlot g_front, g_back;      // captured halves of the pipeline (batch path)
+    // Fixed scratch (alls_assemble_noalloc): when set, the list pool and
+    // consolidation arrays are carved from it, the fused pass is not used
+    // and the output CSR is the caller's (checked, never reallocated).
+    char* ext = nullptr;
+    size_t ext_bytes = 0;
     hipError_t init();
     void destroy();
     size_t device_bytes() const;

@@ -148,6 +148,23 @@ int         alls_device_count(void);
  * (Acc:329-380) computes on that pattern. */
 alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts, alls_csr* out);
 
+/* alls_assemble without any device allocation.  No reference counterpart:
+ * it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The list pool and
+ * consolidation arrays live in a fixed device array of the library, 128 MiB
+ * per GPU, held until the process ends; a mesh whose lists need more
+ * returns ALLS_E_UNSUPPORTED (about 100k Q4 elements, 50k triangles or 1,500
+ * Hex20 elements fit).  The mesh arrays must be device memory (mem =
+ * ALLS_MEM_DEVICE, 16-byte aligned); opts->autotune must be 0.  `out` holds
+ * the caller's device buffers: row_ptr (cap_rows entries, int64 when
+ * row_ptr_is_i64 is set, else int32), col_ind and val (cap_nnz entries);
+ * they are filled and n_rows/nnz set, never reallocated or freed.  If they
+ * are too small the call returns ALLS_E_INVALID with out->n_rows and
+ * out->nnz set to the sizes needed.  Every call on one GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (two contexts, two threads)
+ * race.  Values are bitwise those of alls_assemble. */
+alls_status alls_assemble_noalloc(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts, alls_csr* out);
+
 /* Copies a CSR to host memory.  row_ptr is always returned as int64.  Any
  * of the three destinations may be NULL. */
 alls_status alls_csr_download(alls_ctx* ctx, const alls_csr* csr, int64_t* row_ptr,
