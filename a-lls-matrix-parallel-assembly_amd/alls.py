@@ -61,6 +61,14 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerCounts(C.Structure):
+    _fields_ = [
+        ("completed", C.c_int64), ("batched", C.c_int64), ("passed", C.c_int64),
+        ("builds", C.c_int64 * 2), ("hits", C.c_int64 * 2), ("evictions", C.c_int64 * 2),
+        ("evicted_unused", C.c_int64 * 2),
+    ]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("n_streams", C.c_int32), ("hbm_budget_bytes", C.c_int64),
@@ -130,6 +138,7 @@ SIGNATURES = {
     "alls_server_submit": (C.c_int, [vp, P(Mesh), C.c_uint64, P(Opts), P(C.c_int64)]),
     "alls_server_wait": (C.c_int, [vp, C.c_int64, P(Csr), P(RequestStats)]),
     "alls_server_release": (None, [vp, P(Csr)]),
+    "alls_server_counters": (C.c_int, [vp, P(ServerCounts)]),
     "alls_server_destroy": (None, [vp]),
     "alls_gen_mesh": (C.c_int, [P(GenSpec), P(HostMesh)]),
     "alls_host_mesh_free": (None, [P(HostMesh)]),
@@ -664,6 +673,16 @@ class Server:
 
     def release(self, csr):
         lib().alls_server_release(self.handle, C.byref(csr))
+
+    def counters(self):
+        """alls_server_counters: what the server has done ([2D, Hex20] pairs)."""
+        c = ServerCounts()
+        check(lib().alls_server_counters(self.handle, C.byref(c)))
+        out = {}
+        for k, t in ServerCounts._fields_:
+            v = getattr(c, k)
+            out[k] = list(v) if hasattr(v, "__len__") else v
+        return out
 
     def close(self):
         if self.handle:

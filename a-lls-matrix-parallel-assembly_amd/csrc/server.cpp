@@ -4,18 +4,24 @@
 // the legacy stream (Acc:750-893).  Here n worker threads each own a HIP
 // stream and a workspace.  A request is admitted when a worker is idle and its
 // HBM footprint fits the remaining budget, so requests that arrive while
-// others run start (in arrival order) as soon as a slot and memory free up,
-// and their kernels overlap on the device.  Requests that carry a mesh key can
-// keep that mesh's pattern on the device (LRU-evicted within the budget), so
-// a repeat request with a new E field runs only the value-only path.
+// others run start as soon as a slot and memory free up, and their kernels
+// overlap on the device.  The general workers take the queue in arrival
+// order; with two or more workers the last one is an express lane for small
+// requests, which may pass a larger request at the head of the queue while
+// that head has waited less than an aging window (so the head is never
+// starved).  Requests that carry a mesh key can keep that mesh's pattern on
+// the device, so a repeat request with a new E field runs only the
+// value-only path.
 //
 // Building a pattern costs more than a plain assembly and a cached repeat
 // saves only part of one (A-LLS itself is close to the value-only path), so
-// by default a pattern is built only when the key's history pays for it:
-// with max(c, 1) expected future repeats for a key seen c times before,
-// build when repeats * (plain - hit) > build - plain, using per-element
-// costs of each path measured by the server itself (running averages per
-// dimension, seeded with typical MI355X figures).
+// by default a pattern is built only when the key's expected future repeats
+// pay for it.  Those are read from the server's history: of the keys (per
+// dimension) seen as often as this one, how many more times they came back.
+// They are scaled by the share of predicted hits that earlier patterns
+// delivered before they were evicted, and the costs are per-element running
+// averages of each path the server measures itself.  Patterns are evicted
+// lowest expected saving per byte first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,6 +60,8 @@ struct Request {
     bool cache_hit = false;
     bool built = false;
     int batch_size = 1;
+    int running_at_admit = 0;       // requests already on the device when it was admitted
+    uint64_t admit_index = 0;       // the server's admission count including it
     alls_csr out{};
     std::string err;
 };
@@ -63,6 +71,8 @@ struct CacheEntry {
     int64_t bytes = 0;
     int refs = 0;
     uint64_t last_use = 0;
+    int64_t hits = 0;               // value-only assemblies served
+    double predicted = 0.0;         // expected repeats when it was built
     int precision = 0;
     int64_t n_elems = 0, n_nodes = 0;
     // What a value-only repeat takes from the cached pattern besides the
@@ -171,22 +181,62 @@ struct alls_server {
     int64_t next_ticket = 1;
     uint64_t use_clock = 0;
     std::unordered_map<uint64_t, int> seen;   // admitted requests per mesh key
-    // Per-element device cost (ns) of the plain, build and hit paths, per
-    // dimension (0: 2D, 1: Hex20), as running averages of served requests.
-    double cost[2][3] = {{0.7, 1.2, 0.35}, {35.0, 120.0, 30.0}};
+    // ge[d][j]: keys of dimension d (0: 2D, 1: Hex20) admitted at least j times.
+    std::vector<int64_t> ge[2];
+    // Hits delivered and hits predicted-but-lost by patterns already evicted.
+    double kept[2] = {0.0, 0.0}, lost[2] = {0.0, 0.0};
+    alls_server_counts counts{};
+    // Per-element cost (ns, admission to completion, host inputs staged) of
+    // the plain, build and hit paths, per dimension (0: 2D, 1: Hex20), as
+    // running averages of requests that ran alone on the device: times taken
+    // while other requests share it measure the contention, not the path, and
+    // once made the Hex20 builds look worth it on a saturated device.  Seeded
+    // with the levels measured on the served mix.
+    double cost[2][3] = {{5.0, 6.5, 2.5}, {40.0, 150.0, 36.0}};
+    int running = 0;                 // requests on the device
+    uint64_t admits = 0;             // admissions so far
 
     enum { PATH_PLAIN = 0, PATH_BUILD = 1, PATH_HIT = 2 };
-    // Whether a request of a key seen `prior` times should build its pattern:
-    // only a key that has come back (prior >= 1), and only when its expected
-    // repeats (as many as it has had) pay for the build.  Under concurrency
-    // the measured build and plain costs come out close, so without the
-    // first condition every new key would build; in the served mix most keys
-    // never repeat and their patterns were evicted unused.
-    bool worth_building(const Request& q, int prior) const {
+    static int dim_of(const Request& q) { return q.mesh.dim == 3 ? 1 : 0; }
+
+    // Further requests expected for a key admitted `k` times so far: of the
+    // keys of this dimension that reached k requests, the mean number of
+    // requests they had beyond k (this undercounts early in a run, when keys
+    // have had little time to come back, which only makes the policy
+    // cautious).  Until kHistMin keys have reached k, the key's own history
+    // stands in: its k - 1 earlier requests predict as many more.
+    static constexpr int64_t kHistMin = 8;
+    double expected_repeats(int d, int k) const {
+        if (k < 1) return 0.0;
+        const std::vector<int64_t>& g = ge[d];
+        if (k < (int)g.size() && g[k] >= kHistMin) {
+            double more = 0.0;
+            for (size_t j = (size_t)k + 1; j < g.size(); ++j) more += (double)g[j];
+            return more / (double)g[k];
+        }
+        return (double)(k - 1);
+    }
+    // Share of predicted hits that evicted patterns delivered (1 until some
+    // pattern has been evicted).
+    double survival(int d) const { return (kept[d] + 1.0) / (kept[d] + lost[d] + 1.0); }
+    // Whether the request (the key's k-th, counting itself) should build its
+    // pattern; *pred receives the repeats it expects.
+    bool worth_building(const Request& q, int k, double* pred) const {
+        const int d = dim_of(q);
+        const double e = expected_repeats(d, k);
+        if (pred) *pred = e;
         if (cfg.cache_policy == ALLS_CACHE_EAGER) return true;
-        if (prior < 1) return false;
-        const double* c = cost[q.mesh.dim == 3 ? 1 : 0];
-        return prior * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
+        const double* c = cost[d];
+        return e * survival(d) * (c[PATH_PLAIN] - c[PATH_HIT]) > c[PATH_BUILD] - c[PATH_PLAIN];
+    }
+    // Records an admission of key `key` (dimension d); returns the key's
+    // request count including this one.
+    int note_seen(uint64_t key, int d) {
+        const int k = ++seen[key];
+        std::vector<int64_t>& g = ge[d];
+        if ((int)g.size() <= k) g.resize((size_t)k + 1, 0);
+        g[k]++;
+        return k;
     }
     // Small plain requests queued right behind a plain head join it in one
     // packed batch (one pipeline, persistent insertion, split afterwards).
@@ -200,21 +250,31 @@ struct alls_server {
                (a.mesh.dim == 3 || a.mesh.thickness == b.mesh.thickness) &&
                (a.mesh.dim == 2 || a.opts.precision == b.opts.precision);
     }
+    // Express lane: requests whose plain assembly takes at most kSmallMs at
+    // the measured cost; they may pass a head that has waited < kAgeMs.
+    static constexpr double kSmallMs = 2.0;
+    static constexpr double kAgeMs = 40.0;
+    bool small(const Request& q) const {
+        return (double)q.mesh.n_elems * cost[dim_of(q)][PATH_PLAIN] * 1e-6 <= kSmallMs;
+    }
     // Whether queued request q would run the plain path if it were the head.
     bool runs_plain(const Request& q) const {
         if (!q.key) return true;
         auto c = cache.find(q.key);
         if (c != cache.end() && c->second.serves(q.mesh, q.opts)) return false;
         auto sc = seen.find(q.key);
-        return building.count(q.key) || !worth_building(q, sc == seen.end() ? 0 : sc->second);
+        const int k = (sc == seen.end() ? 0 : sc->second) + 1;
+        return building.count(q.key) || !worth_building(q, k, nullptr);
     }
 
-    // Folds one served request into the running per-element cost; small
-    // requests are dominated by fixed costs and are left out.
+    // Folds one served request into the running per-element cost.  Small
+    // requests are dominated by fixed costs and are left out, and so is any
+    // request that shared the device with another.
     void observe(const Request& r, int path) {
         if (r.mesh.n_elems < 50000 || r.status != ALLS_OK) return;
+        if (r.running_at_admit != 0 || admits != r.admit_index || running != 1) return;
         const double ns = ms_between(r.t_admit, r.t_done) * 1e6 / (double)r.mesh.n_elems;
-        double& c = cost[r.mesh.dim == 3 ? 1 : 0][path];
+        double& c = cost[dim_of(r)][path];
         c = 0.8 * c + 0.2 * ns;
     }
     bool stop = false;
@@ -228,17 +288,59 @@ struct alls_server {
         for (auto& kv : cache) b += kv.second.bytes;
         return b;
     }
-    // Frees least-recently used idle patterns until `need` bytes fit.
+    // Drops a cached pattern (idle), booking its delivered and lost hits.
+    void drop(std::unordered_map<uint64_t, CacheEntry>::iterator it) {
+        const int d = it->second.dim == 3 ? 1 : 0;
+        counts.evictions[d]++;
+        if (it->second.hits == 0) counts.evicted_unused[d]++;
+        kept[d] += (double)it->second.hits;
+        lost[d] += std::max(0.0, it->second.predicted - (double)it->second.hits);
+        pattern_destroy(it->second.pat);
+        cache.erase(it);
+    }
+    // Expected saving (ns) of keeping a pattern, per byte it holds.
+    double keep_value(uint64_t key, const CacheEntry& c) const {
+        const int d = c.dim == 3 ? 1 : 0;
+        auto sc = seen.find(key);
+        const double e = expected_repeats(d, sc == seen.end() ? 0 : sc->second);
+        return e * (cost[d][PATH_PLAIN] - cost[d][PATH_HIT]) * (double)c.n_elems / (double)std::max<int64_t>(c.bytes, 1);
+    }
+    // Frees idle patterns, lowest expected saving per byte first (the least
+    // recently used among equals), until `need` bytes fit.
     void evict_for(int64_t need) {
         while (in_use + ws_total + cache_bytes() + need > budget) {
             auto victim = cache.end();
-            for (auto it = cache.begin(); it != cache.end(); ++it)
-                if (it->second.refs == 0 && (victim == cache.end() || it->second.last_use < victim->second.last_use))
+            double vv = 0.0;
+            for (auto it = cache.begin(); it != cache.end(); ++it) {
+                if (it->second.refs != 0) continue;
+                const double v = keep_value(it->first, it->second);
+                if (victim == cache.end() || v < vv ||
+                    (v == vv && it->second.last_use < victim->second.last_use)) {
                     victim = it;
+                    vv = v;
+                }
+            }
             if (victim == cache.end()) return;
-            pattern_destroy(victim->second.pat);
-            cache.erase(victim);
+            drop(victim);
         }
+    }
+
+    // Index of the queued request worker `slot` may start, or -1.  General
+    // workers take the head.  The express worker takes only small requests:
+    // the head if it is small, else the first small one among the next 64
+    // while the head has waited less than kAgeMs (after that it waits, so
+    // that memory frees up for the head).  Letting it take large heads as
+    // well measured much worse on the served mix (two large Hex20 requests at
+    // once hold two list pools and push the cached patterns out).
+    int pick(int slot) const {
+        if (queue.empty()) return -1;
+        const bool express = cfg.n_streams >= 2 && slot == cfg.n_streams - 1;
+        if (!express || small(*queue.front())) return 0;
+        if (ms_between(queue.front()->t_submit, Clock::now()) >= kAgeMs) return -1;   // the head's turn
+        const int lim = (int)std::min<size_t>(queue.size(), 64);
+        for (int i = 1; i < lim; ++i)
+            if (small(*queue[i])) return i;
+        return -1;
     }
 
     void worker(int slot) {
@@ -251,25 +353,25 @@ struct alls_server {
             std::vector<std::shared_ptr<Request>> batch;   // plain requests batched behind r
             bool hit = false, build = false;
             CacheEntry* ce = nullptr;
+            double predicted = 0.0;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 for (;;) {
                     if (stop) return;
-                    // Strict FIFO: only the oldest request may start.  Letting later
-                    // requests pass a head that does not fit could starve it forever:
-                    // their outputs stay charged until the caller waits for them, and a
-                    // caller that waits in submission order is blocked on the head.
                     // Two tries: the second after this idle worker gives its
-                    // workspace back, so held scratch can never starve the head.
-                    for (int attempt = 0; attempt < 2 && !r && !queue.empty(); ++attempt) {
-                        std::shared_ptr<Request> q = queue.front();
+                    // workspace back, so held scratch can never starve a request.
+                    for (int attempt = 0; attempt < 2 && !r; ++attempt) {
+                        const int qi = pick(slot);
+                        if (qi < 0) break;
+                        std::shared_ptr<Request> q = queue[(size_t)qi];
                         auto c = q->key ? cache.find(q->key) : cache.end();
                         const bool can_hit = c != cache.end() && c->second.serves(q->mesh, q->opts);
                         const auto sc = q->key ? seen.find(q->key) : seen.end();
-                        const int prior = sc == seen.end() ? 0 : sc->second;
+                        const int k = (sc == seen.end() ? 0 : sc->second) + 1;
+                        double pred = 0.0;
                         const bool can_build = q->key && !can_hit && !building.count(q->key) &&
                                                (c == cache.end() || c->second.refs == 0) &&
-                                               worth_building(*q, prior);
+                                               q->pattern_bytes <= budget / 4 && worth_building(*q, k, &pred);
                         // This worker's workspace is reused; only its growth is new memory.
                         const int64_t grow = std::max<int64_t>(0, q->scratch - (int64_t)w.device_bytes());
                         const int64_t need = (can_build ? q->footprint : q->footprint - q->pattern_bytes) + grow;
@@ -280,15 +382,16 @@ struct alls_server {
                         if (in_use + ws_total + cache_bytes() - evictable + need <= budget) {
                             evict_for(need);
                             auto stale = can_build ? cache.find(q->key) : cache.end();
-                            if (stale != cache.end()) {   // idle entry of another mesh under this key
-                                pattern_destroy(stale->second.pat);
-                                cache.erase(stale);
-                            }
+                            if (stale != cache.end()) drop(stale);   // idle entry of another mesh under this key
                             r = q;
-                            queue.pop_front();
-                            if (r->key) seen[r->key]++;
+                            queue.erase(queue.begin() + qi);
+                            r->running_at_admit = running++;
+                            r->admit_index = ++admits;
+                            if (qi > 0) counts.passed++;
+                            if (r->key) note_seen(r->key, dim_of(*r));
                             hit = can_hit;
                             build = can_build;
+                            predicted = pred;
                             if (hit) {
                                 ce = &cache[r->key];
                                 ce->last_use = ++use_clock;
@@ -307,12 +410,14 @@ struct alls_server {
                         for (int64_t b : ws_bytes) ws_total += b;
                     }
                     if (r) break;
-                    cv_work.wait(lk);
+                    cv_work.wait_for(lk, std::chrono::milliseconds(5));   // the express lane re-checks ages
                 }
                 if (!hit && !build && batchable(*r)) {
+                    // Compatible small plain requests queued right behind r join it.
                     int64_t elems = r->mesh.n_elems;
-                    while (!queue.empty() && (int)batch.size() + 1 < kBatchMax) {
-                        std::shared_ptr<Request> q = queue.front();
+                    size_t i = 0;
+                    while (i < queue.size() && (int)batch.size() + 1 < kBatchMax) {
+                        std::shared_ptr<Request> q = queue[i];
                         if (!batchable(*q) || !compatible(*r, *q) || !runs_plain(*q) ||
                             elems + q->mesh.n_elems > batch_elems_max(r->mesh.dim))
                             break;
@@ -321,8 +426,9 @@ struct alls_server {
                         const int64_t out_b = q->footprint - q->pattern_bytes;
                         const int64_t need = 2 * out_b + q->scratch;
                         if (in_use + ws_total + cache_bytes() + need > budget) break;
-                        queue.pop_front();
-                        if (q->key) seen[q->key]++;
+                        queue.erase(queue.begin() + (std::ptrdiff_t)i);
+                        ++admits;   // batched along: r no longer ran alone
+                        if (q->key) note_seen(q->key, dim_of(*q));
                         q->footprint = need;
                         in_use += need;
                         elems += q->mesh.n_elems;
@@ -358,6 +464,7 @@ struct alls_server {
                     e.pat = p;
                     e.bytes = pattern_bytes(p);
                     e.last_use = ++use_clock;
+                    e.predicted = predicted;
                     e.describe(r->mesh, r->opts);
                     if ((int)cache.size() >= std::max(1, cfg.pattern_cache_entries)) {
                         auto victim = cache.end();
@@ -365,12 +472,10 @@ struct alls_server {
                             if (it->second.refs == 0 &&
                                 (victim == cache.end() || it->second.last_use < victim->second.last_use))
                                 victim = it;
-                        if (victim != cache.end()) {
-                            pattern_destroy(victim->second.pat);
-                            cache.erase(victim);
-                        }
+                        if (victim != cache.end()) drop(victim);
                     }
                     cache[r->key] = e;
+                    counts.builds[dim_of(*r)]++;
                 }
             } else if (!batch.empty()) {
                 batch.insert(batch.begin(), r);
@@ -383,6 +488,10 @@ struct alls_server {
                 if (rc == ALLS_OK) rc = stage_mesh(w, &r->mesh, mi, &dm, s, &r->err);
                 if (rc == ALLS_OK) rc = run_alls(w, dm, mi, r->opts, &r->out, s, &st, &r->err);
             }
+            // A workspace grown past an eighth of the budget (the list pool
+            // of a large Hex20 request) is given back: held, it would push
+            // cached patterns out, and the arena carves it again quickly.
+            if ((int64_t)w.device_bytes() > budget / 8) w.release_device();
             {
                 std::lock_guard<std::mutex> lk(mu);
                 ws_bytes[slot] = (int64_t)w.device_bytes();
@@ -391,10 +500,19 @@ struct alls_server {
                 r->t_done = Clock::now();
                 r->status = rc;
                 if (batch.empty()) observe(*r, hit ? PATH_HIT : build ? PATH_BUILD : PATH_PLAIN);
+                --running;
                 r->cache_hit = hit;
                 r->built = build && rc == ALLS_OK;
                 r->done = true;
-                if (hit) ce->refs--;
+                if (hit) {
+                    ce->refs--;
+                    if (rc == ALLS_OK) {
+                        ce->hits++;
+                        counts.hits[dim_of(*r)]++;
+                    }
+                }
+                counts.completed += 1 + (int64_t)batch.size();
+                if (!batch.empty()) counts.batched += 1 + (int64_t)batch.size();
                 // The output stays charged until the caller releases it.
                 const int64_t outb = r->out.cap_nnz * 12 + r->out.cap_rows * 8;
                 in_use -= r->footprint;
@@ -517,6 +635,13 @@ alls_status alls_server_wait(alls_server* s, int64_t ticket, alls_csr* out, alls
         s->cv_work.notify_all();
     }
     return r->status;
+}
+
+alls_status alls_server_counters(alls_server* s, alls_server_counts* out) {
+    if (!s || !out) return ALLS_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->counts;
+    return ALLS_OK;
 }
 
 void alls_server_release(alls_server* s, alls_csr* csr) {

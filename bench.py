@@ -452,7 +452,7 @@ def serve_costs_from(res, extra, n_q4):
     return c
 
 
-def served_mix(n_requests, costs):
+def served_mix(n_requests, costs, streams=2):
     """Poisson mix of Q4/T3/Hex20 requests (1e3-1e6 elements, 60% mesh reuse).
     The arrival rate puts the offered load at 75% of one stream's measured
     service rate: a separate sample of 200 requests from the same
@@ -465,17 +465,24 @@ def served_mix(n_requests, costs):
     meshes, reqs = serve_mix.build_requests(n_requests, progress=log)
     _, cal = serve_mix.build_requests(200, seed=0xA11500C6)
     t_gen = time.perf_counter() - t0
-    with alls.Server(n_streams=2) as srv:
+    with alls.Server(n_streams=streams) as srv:
         per_req_ms = serve_mix.one_stream_ms(srv, cal)
     rate = 0.75 * 1e3 / per_req_ms
     busy_s = serve_mix.offered_busy_s(reqs, costs)
-    with alls.Server(n_streams=2) as srv:
+    with alls.Server(n_streams=streams) as srv:
         stats, makespan, errors = serve_mix.run(srv, reqs, rate, progress=log)
+        counters = srv.counters()
     res = serve_mix.summarize(reqs, stats, makespan, errors)
+    res["server_counters"] = {
+        "builds": {"2d": counters["builds"][0], "hex20": counters["builds"][1]},
+        "hits": {"2d": counters["hits"][0], "hex20": counters["hits"][1]},
+        "evictions": {"2d": counters["evictions"][0], "hex20": counters["evictions"][1]},
+        "evicted_unused": {"2d": counters["evicted_unused"][0], "hex20": counters["evicted_unused"][1]},
+        "batched": counters["batched"], "passed_by_express_lane": counters["passed"]}
     res.update({"arrival_rate_per_s": round(rate, 1), "unique_meshes": len(meshes),
                 "one_stream_ms_per_request": round(per_req_ms, 3),
                 "model_busy_s": round(busy_s, 2), "cost_model_ms_per_element": costs,
-                "generation_s": round(t_gen, 1), "streams": 2,
+                "generation_s": round(t_gen, 1), "streams": streams,
                 "workload": "Poisson arrivals; type uniform over {Q4, T3, Hex20}; size log-uniform 1e3-1e6 "
                             "elements; 60% reuse a seen mesh with a new lognormal(0,0.5) E (cached pattern)"})
     return res
@@ -496,6 +503,7 @@ def main():
     ap.add_argument("--extras", default="coloring,autotune,batch,c3,hex20,serve",
                     help="comma list of secondary measurements to run (or 'none')")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-streams", type=int, default=2, help="server workers (the last is the express lane)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="end with one device-to-device copy of BYTES (a copy kernel of known size, the "
                          "scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -595,7 +603,7 @@ def main():
         try:
             log(f"served mix: {a.serve_requests} requests")
             costs = dict(serve_costs_from(res, extra, mesh.n_elems), fixed=0.2)
-            extra["served_mix"] = served_mix(a.serve_requests, costs)
+            extra["served_mix"] = served_mix(a.serve_requests, costs, a.serve_streams)
         except Exception as e:
             extra["served_mix"] = {"error": str(e)}
     if extra:

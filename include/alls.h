@@ -246,10 +246,13 @@ alls_status alls_autotune(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
  * the budget, run concurrently on up to n_streams streams, and reuse cached
  * patterns when mesh_key repeats. */
 /* When a request with a mesh key builds and caches its pattern.  COST (the
- * default): once the key has come back, when the expected repeats (the
- * key's earlier requests) times the saving of a cached repeat exceed the
- * extra cost of the build, with per-element costs the server measures per
- * path and dimension.  EAGER: on the first request of every key. */
+ * default): when the key's expected further repeats, times the share of
+ * predicted hits that cached patterns have delivered before eviction, times
+ * the saving of a cached repeat exceed the extra cost of the build.  The
+ * expected repeats of a key seen k times are read from the server's own
+ * history (keys seen k times that came back, per dimension); the costs are
+ * per-element running averages the server measures per path and dimension.
+ * EAGER: on the first request of every key. */
 typedef enum { ALLS_CACHE_COST = 0, ALLS_CACHE_EAGER = 1 } alls_cache_policy;
 
 typedef struct {
@@ -273,7 +276,21 @@ typedef struct {
     int32_t reserved;
 } alls_request_stats;
 
+/* What a server has done so far (alls_server_counts).  Index 0 counts 2D
+ * requests, index 1 Hex20. */
+typedef struct {
+    int64_t completed;        /* requests finished (any path, any status) */
+    int64_t batched;          /* of them, assembled in a packed batch with others */
+    int64_t passed;           /* admitted ahead of an older queued request (express lane) */
+    int64_t builds[2];        /* patterns built and cached */
+    int64_t hits[2];          /* value-only assemblies on a cached pattern */
+    int64_t evictions[2];     /* cached patterns dropped to make room (or replaced under their key) */
+    int64_t evicted_unused[2];/* of them, dropped before any hit */
+} alls_server_counts;
+
 alls_status alls_server_create(const alls_server_config* cfg, alls_server** srv);
+/* Copies the server's counters into *out. */
+alls_status alls_server_counters(alls_server* srv, alls_server_counts* out);
 /* Enqueues a request; the mesh arrays must stay valid until it completes.
  * mesh_key != 0 enables the pattern cache for this mesh.  A key promises the
  * same node coordinates and connectivity on every request that carries it:

@@ -63,11 +63,12 @@ def test_value_only_hits_are_bitwise_repeatable():
 
 
 def test_cost_policy_builds_only_when_repeats_pay_for_it():
-    # Default policy with the seeded 2D costs (plain 0.7, build 1.2, hit 0.35
-    # ns/element; small requests do not update them): a key builds its
-    # pattern once 2 earlier requests predict enough repeats (2 * 0.35 > 0.5;
-    # a key seen for the first time never builds), and hits after that.
-    # Every result equals the plain assembly.
+    # Default policy with the seeded 2D costs (plain 5.0, build 6.5, hit 2.5
+    # ns/element; small requests do not update them) and a lone key (too
+    # little history, so the key's own requests predict its repeats): the
+    # first request never builds (no repeat expected), the second builds since
+    # 1 * (5.0 - 2.5) > 6.5 - 5.0, and every later one hits.  Every result
+    # equals the plain assembly.
     m = alls.generate(alls.GEN_Q4, 30, 20, jitter=0.2, seed=5)
     rng = np.random.default_rng(3)
     kinds, outs = [], []
@@ -78,7 +79,7 @@ def test_cost_policy_builds_only_when_repeats_pay_for_it():
             kinds.append(st["cache_hit"])
             outs.append((mk, srv.download(csr)))
             srv.release(csr)
-    assert kinds == [0, 0, 2, 1, 1, 1, 1]
+    assert kinds == [0, 2, 1, 1, 1, 1, 1]
     for mk, got in outs:
         compare_csr(got, oracle.assemble(mk))
 
