@@ -54,7 +54,7 @@ class Stats(C.Structure):
         ("n_elems", C.c_int64), ("n_rows", C.c_int64), ("nnz", C.c_int64),
         ("list_entries", C.c_int64), ("pool_chunks_used", C.c_int64), ("pool_chunks_cap", C.c_int64),
         ("scratch_bytes", C.c_int64), ("chunk_slots", C.c_int32), ("retries", C.c_int32),
-        ("n_colors", C.c_int32), ("big_rows", C.c_int32),
+        ("n_colors", C.c_int32), ("big_rows", C.c_int32), ("graph_launches", C.c_int32), ("_pad", C.c_int32),
     ]
 
     def as_dict(self):

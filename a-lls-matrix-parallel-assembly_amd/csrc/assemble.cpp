@@ -371,15 +371,13 @@ size_t Workspace::device_bytes() const {
 }
 
 void Workspace::release_device() {
-    g_front.reset();
-    g_back.reset();
+    graph.reset();
     for (DevBuf* b : {&hdr, &keys, &geid, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b->release();
 }
 
 void Workspace::destroy() {
-    g_front.reset();
-    g_back.reset();
+    graph.reset();
     for (DevBuf* b : {&hdr, &keys, &geid, &vals, &misc, &u, &big, &tiles, &coords, &conn, &types, &E, &bdesc, &bsplit})
         b->release();
     if (host_misc) (void)hipHostFree(host_misc);
@@ -713,19 +711,23 @@ static alls_status enqueue_insert(Workspace& ws, const DevMesh& dm, const MeshIn
 }
 
 // Count pass, scan totals and the read-back of the flags and the total.
+static alls_status enqueue_readback(Workspace& ws, const Pipeline& pl, hipStream_t s, std::string* err) {
+    HIP_TRY(hipMemcpyAsync(ws.host_misc, pl.lp.misc, MISC_SIZE * sizeof(int32_t), hipMemcpyDeviceToHost, s),
+            "read flags");
+    HIP_TRY(hipMemcpyAsync(ws.host_misc + 32, ws.tiles.as<int64_t>() + pl.ntiles, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, s), "read total");
+    return ALLS_OK;
+}
+
 static alls_status enqueue_count(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const Pipeline& pl,
-                                 hipStream_t s, bool events, std::string* err) {
+                                 hipStream_t s, bool events, std::string* err, bool readback = true) {
     launch_count(pl.lp, pl.G, mi.f32 ? 1 : 0, ws.u.as<int32_t>(), ws.big.as<int32_t>(), s);
     HIP_TRY(hipGetLastError(), "launch count");
     if (events) HIP_TRY(hipEventRecord(ws.ev[3], s), "event");
     launch_scan_total(ws.u.as<int32_t>(), dm.n_nodes, ws.tiles.as<int64_t>(), pl.ntiles, s);
     HIP_TRY(hipGetLastError(), "launch scan");
     if (events) HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
-    HIP_TRY(hipMemcpyAsync(ws.host_misc, pl.lp.misc, MISC_SIZE * sizeof(int32_t), hipMemcpyDeviceToHost, s),
-            "read flags");
-    HIP_TRY(hipMemcpyAsync(ws.host_misc + 32, ws.tiles.as<int64_t>() + pl.ntiles, sizeof(int64_t),
-                           hipMemcpyDeviceToHost, s), "read total");
-    return ALLS_OK;
+    return readback ? enqueue_readback(ws, pl, s, err) : ALLS_OK;
 }
 
 // Header reset, integration + insertion, count, scan totals, and the
@@ -925,6 +927,86 @@ static alls_status graph_launch(GraphSlot& g, uint64_t key, hipStream_t s, F&& e
     return ALLS_OK;
 }
 
+// The whole pipeline as one captured graph (batch path).  The output is
+// sized by the entries bound (each list entry is at most one new column), so
+// no host read-back splits the pipeline, and the write pass's long-row
+// kernels are captured for rows of up to `cap` entries: the list directory's
+// bound in 2D (at most 256), three 60-entry segment slots for Hex20 (a
+// conforming brick's rows hold 160).  After the graph, one read-back of the
+// flags and the total: a list overflow rebuilds the lists and captures again;
+// a row longer than the captured kernels take (flagged, or the count pass's
+// longest row above `cap`) is written again, uncaptured, with the real bound.
+static alls_status run_graph(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, Pipeline& pl,
+                             alls_csr* out, hipStream_t s, alls_stats* st, std::string* err) {
+    const int dpn = mi.dpn;
+    const int64_t n_rows = dm.n_nodes * dpn;
+    const int64_t bound = std::max<int64_t>(mi.entries_bound, 1) * dpn * dpn;
+    const int i64 = (o.row_ptr_i64 || bound > (int64_t)INT32_MAX) ? 1 : 0;
+    int launches = 0;
+    int32_t cap = 0;
+    for (;;) {
+        alls_status rs = alloc_lists(ws, dm, mi, pl, err);
+        if (rs != ALLS_OK) return rs;
+        rs = ensure_csr(out, n_rows, bound, i64);
+        if (rs != ALLS_OK) {
+            if (err) *err = "cannot allocate output CSR";
+            return rs;
+        }
+        cap = mi.kind == 3 ? 180 : std::min(ALLS_DIR * pl.C, 256);
+        pl.max_big = cap;
+        HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
+        rs = graph_launch(ws.graph, mix(front_key(ws, dm, mi, o, pl), back_key(ws, dm, mi, pl, out)), s,
+                          [&] {
+                              alls_status r = enqueue_insert(ws, dm, mi, o, pl, s, false, err);
+                              if (r == ALLS_OK) r = enqueue_count(ws, dm, mi, pl, s, false, err, false);
+                              if (r == ALLS_OK) r = enqueue_back(ws, dm, mi, pl, out, s, false, err);
+                              if (r == ALLS_OK) r = enqueue_readback(ws, pl, s, err);
+                              return r;
+                          },
+                          err);
+        if (rs != ALLS_OK) return rs;
+        ++launches;
+        HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+        HIP_TRY(hipStreamSynchronize(s), "sync after graph");
+        int32_t* f = reinterpret_cast<int32_t*>(ws.host_misc);
+        const int32_t too_big = f[MISC_ERR + ERR_ROW_TOO_BIG];
+        f[MISC_ERR + ERR_ROW_TOO_BIG] = 0;   // a write-pass flag here, not a list overflow
+        bool retry = false;
+        rs = check_front(ws, pl, &retry, err);
+        if (rs != ALLS_OK) return rs;
+        if (retry) continue;
+        if (too_big || pl.max_big > cap) {
+            // Rows the captured write kernels could not take: write again.
+            rs = enqueue_back(ws, dm, mi, pl, out, s, false, err);
+            if (rs != ALLS_OK) return rs;
+            HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+            HIP_TRY(hipStreamSynchronize(s), "sync after write");
+        }
+        break;
+    }
+    out->n_rows = n_rows;
+    out->nnz = pl.total * dpn * dpn;
+    out->pattern_id = 0;
+    if (st) {
+        *st = alls_stats{};
+        st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);
+        st->ms_write = st->ms_total;
+        st->n_elems = dm.n_elems;
+        st->n_rows = n_rows;
+        st->nnz = out->nnz;
+        st->list_entries = mi.entries_bound;
+        st->chunk_slots = pl.C;
+        st->retries = pl.retries;
+        st->scratch_bytes = (int64_t)(ws.hdr.bytes + ws.keys.bytes + ws.geid.bytes + ws.vals.bytes);
+        const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
+        st->big_rows = f[MISC_BIG_COUNT];
+        st->pool_chunks_used = dm.n_nodes + f[MISC_OVF_TOP];
+        st->pool_chunks_cap = pl.cap_chunks;
+        st->graph_launches = launches;
+    }
+    return ALLS_OK;
+}
+
 alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
                      hipStream_t s, alls_stats* st, std::string* err, bool use_graph) {
     HIP_TRY(ws.init(), "workspace init");
@@ -937,16 +1019,15 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     if (mi.kind == 0) pl.C = std::max(4, pl.C / 4 * 4);   // whole slot groups per chunk
     pl.G = small_group(mi.kind);
     pl.ntiles = scan_tile_count(n_nodes);
-    // Graph replays do not record the per-stage events; the whole front and
-    // back halves are timed around the launches instead.
-    const bool stage_events = !use_graph;
+    if (use_graph) return run_graph(ws, dm, mi, o, pl, out, s, st, err);
+    const bool stage_events = true;
     // T3 and mixed meshes (32-lane rows) whose rows all fit the register
     // kernel take the fused pass (count + scan + write in one launch), into an
     // output sized by the entries bound; if a row turns out longer, the
     // separate passes run on the same lists.  Measured on MI355X, the fused
     // pass saves 13% of the T3/mixed path, while for Q4 (16-lane rows, a
     // count pass of 0.32 ms at C2) the look-back costs more than it saves.
-    bool try_fused = !use_graph && !ws.ext && pl.G == 32 && n_nodes > 0;
+    bool try_fused = !ws.ext && pl.G == 32 && n_nodes > 0;
     bool fused_done = false;
     for (;;) {
         alls_status rs = alloc_lists(ws, dm, mi, pl, err);
@@ -962,12 +1043,6 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
             launch_fill_i32(pl.lp.misc, MISC_SIZE, 0, s);
             rs = enqueue_count(ws, dm, mi, pl, s, stage_events, err);
             if (rs != ALLS_OK) return rs;
-        } else if (use_graph) {
-            HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
-            rs = graph_launch(ws.g_front, front_key(ws, dm, mi, o, pl), s,
-                              [&] { return enqueue_front(ws, dm, mi, o, pl, s, false, err); }, err);
-            if (rs != ALLS_OK) return rs;
-            HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
         } else {
             rs = enqueue_front(ws, dm, mi, o, pl, s, stage_events, err);
             if (rs != ALLS_OK) return rs;
@@ -1002,12 +1077,6 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     }
     if (fused_done) {
         // nothing left to launch
-    } else if (use_graph) {
-        HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
-        rs = graph_launch(ws.g_back, back_key(ws, dm, mi, pl, out), s,
-                          [&] { return enqueue_back(ws, dm, mi, pl, out, s, false, err); }, err);
-        if (rs != ALLS_OK) return rs;
-        HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
     } else {
         rs = enqueue_back(ws, dm, mi, pl, out, s, stage_events, err);
         if (rs != ALLS_OK) return rs;
@@ -1016,16 +1085,12 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     HIP_TRY(hipStreamSynchronize(s), "sync after write");
     if (st) {
         *st = alls_stats{};
-        if (stage_events) {
-            st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
-            st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
-            st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
-            // The scan's totals, then its application to row_ptr (ahead of the write).
-            st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]) + ev_ms(ws.ev[5], ws.ev[7]);
-            st->ms_write = ev_ms(ws.ev[7], ws.ev[6]);   // merge/write kernels alone
-        } else {
-            st->ms_write = ev_ms(ws.ev[5], ws.ev[6]);   // scan apply + merge/write
-        }
+        st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
+        st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
+        st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
+        // The scan's totals, then its application to row_ptr (ahead of the write).
+        st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]) + ev_ms(ws.ev[5], ws.ev[7]);
+        st->ms_write = ev_ms(ws.ev[7], ws.ev[6]);   // merge/write kernels alone
         st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);   // includes the flag read-back between the halves
         st->n_elems = dm.n_elems;
         st->n_rows = n_rows;

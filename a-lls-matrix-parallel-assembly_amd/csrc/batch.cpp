@@ -5,9 +5,9 @@
 // stage of the A-LLS pipeline is ONE launch over all of them, whatever their
 // sizes, and the result is the block-diagonal CSR of the batch.  The
 // insertion kernel runs persistent (a grid sized to the device, looping over
-// the packed elements) and the stages before and after the pipeline's one
-// host read-back are captured as two hipGraphs and replayed while the batch
-// keeps its shape.  The reference has no batched path: each of its programs
+// the packed elements) and the whole pipeline is captured as ONE hipGraph
+// (its output sized by the list entries bound, so no host read-back splits
+// it) and replayed while the batch keeps its shape.  The reference has no batched path: each of its programs
 // assembles one mesh per call (Acc:750-838, T3:525, Mix:578, Hex:420).
 //
 // Autotuning: block size, persistent grid and list-pool chunk size are

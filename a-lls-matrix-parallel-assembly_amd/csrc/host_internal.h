@@ -89,7 +89,7 @@ struct Workspace {
     hipError_t staging(size_t bytes, char** host);
     hipEvent_t ev[8] = {};
     bool events = false;
-    GraphSlot g_front, g_back;      // captured halves of the pipeline (batch path)
+    GraphSlot graph;                // the captured pipeline (batch path)
     // Fixed scratch (alls_assemble_noalloc): when set, the list pool and
     // consolidation arrays are carved from it, the fused pass is not used
     // and the output CSR is the caller's (checked, never reallocated).
@@ -111,9 +111,10 @@ int64_t pool_chunks(int64_t n_nodes, int64_t entries_bound, int C, int kind, boo
 alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, DevMesh* dm, hipStream_t s,
                        std::string* err);
 // Full A-LLS pipeline on `s`: insert, count, scan, write.  Synchronises `s`.
-// use_graph: the halves before and after the one host read-back run as two
-// hipGraphs, captured on first use and replayed while their arguments (buffer
-// addresses, sizes, material) stay the same.
+// use_graph: the whole pipeline runs as ONE hipGraph into an output sized by
+// the entries bound, captured on first use and replayed while its arguments
+// (buffer addresses, sizes, material) stay the same; the flags and the total
+// are read back once, after it.
 alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
                      hipStream_t s, alls_stats* st, std::string* err, bool use_graph = false);
 // Makes sure `out` can hold n_rows rows and nnz entries.

@@ -111,8 +111,8 @@ typedef struct {
     double ms_insert;         /* element integration + list insertion */
     double ms_count;          /* per-row sort + unique count */
     double ms_scan;           /* device prefix scan -> row_ptr */
-    double ms_write;          /* per-row sort + merge -> col_ind, val (with the row_ptr apply
-                                 when the pipeline ran as a captured graph: no stage events) */
+    double ms_write;          /* per-row sort + merge -> col_ind, val (when the pipeline ran as a
+                                 captured graph: the whole graph, which has no stage events) */
     int64_t n_elems;
     int64_t n_rows;
     int64_t nnz;
@@ -124,6 +124,8 @@ typedef struct {
     int32_t retries;          /* directory-overflow retries taken */
     int32_t n_colors;         /* colored path only */
     int32_t big_rows;         /* rows consolidated by the LDS (wave-per-row) kernel */
+    int32_t graph_launches;   /* hipGraphLaunch calls the call made (batched assembly: 1 per call) */
+    int32_t _pad;
 } alls_stats;
 
 /* ------------------------------------------------------------ context --- */
@@ -216,10 +218,11 @@ alls_status alls_assemble_colored(alls_ctx* ctx, const alls_pattern* pat, const 
  * packed into one node/element space (mesh k's node ids shifted by the nodes
  * of meshes 0..k-1), so each stage is ONE launch over all of them whatever
  * their sizes; the insertion kernel runs persistent (opts->grid_per_cu,
- * default 8 blocks per CU for 2D and 4 for Hex20); and the stages before and
- * after the pipeline's one host read-back run as two hipGraphs, captured on
- * the first call and replayed while the batch keeps its shape (sizes and
- * material layout; new coordinates or E values replay).
+ * default 8 blocks per CU for 2D and 4 for Hex20); and the whole pipeline
+ * runs as ONE hipGraph (into an output sized by the list entries bound, so
+ * no host read-back splits it), captured on the first call and replayed
+ * while the batch keeps its shape (sizes and material layout; new
+ * coordinates or E values replay).
  * The result is the block-diagonal CSR of the batch: mesh k owns rows
  * [row_off[k], row_off[k+1]) and nonzeros [nnz_off[k], nnz_off[k+1]), and
  * its column indices are shifted by row_off[k].  row_off and nnz_off are

@@ -3,7 +3,8 @@
 A batch is assembled in one pipeline over the packed meshes; each mesh's
 block of the block-diagonal result must be bit-identical to assembling that
 mesh alone (same kernels, every column summed in element order), and within
-the oracle tolerance of the CPU oracle.  The captured hipGraphs are replayed
+the oracle tolerance of the CPU oracle.  The captured hipGraph (the whole
+pipeline, one launch per call) is replayed
 several times with new E fields and eager work in between; every replay is
 checked.  Launch settings (autotuned or not) must not change any bit.
 """
@@ -80,8 +81,9 @@ def test_batch_device_meshes(ctx):
 
 
 def test_batch_graph_replays_with_new_E(ctx):
-    # Same shape every call: the graphs are captured once and replayed.  Eager
-    # assemblies on the same context run between the replays.
+    # Same shape every call: the pipeline is captured once as one graph and
+    # replayed, one hipGraphLaunch per call.  Eager assemblies on the same
+    # context run between the replays.
     rng = np.random.default_rng(5)
     meshes = mesh_set_2d()
     out = None
@@ -89,6 +91,7 @@ def test_batch_graph_replays_with_new_E(ctx):
         for m in meshes:
             m.E_per_elem = np.exp(0.5 * rng.standard_normal(m.n_elems))
         out, row_off, nnz_off = ctx.assemble_batch(meshes, out=out)
+        assert ctx.stats()["graph_launches"] == 1
         got = out.download()
         check_batch(ctx, meshes, got, row_off, nnz_off)
         for m, p in zip(meshes[:2], alls.split_batch(got, row_off, nnz_off)[:2]):
@@ -124,3 +127,20 @@ def test_launch_settings_do_not_change_bits(ctx, kind, kw):
     got = alone(ctx, m, autotune=1)
     for a, b in zip(got, ref):
         assert np.array_equal(a, b)
+
+
+def test_batch_row_longer_than_the_captured_write_kernels(ctx):
+    # 300 triangles around one node: a 900-entry row, longer than the 256
+    # entries the captured long-row kernels take.  The graph's write pass
+    # flags it and the write runs again, uncaptured, with the real bound.
+    ang = np.linspace(0, 2 * np.pi, 300, endpoint=False)
+    coords = np.concatenate([[0.0, 0.0], np.column_stack([np.cos(ang), np.sin(ang)]).reshape(-1)])
+    conn = np.array([[0, 1 + i, 1 + (i + 1) % 300] for i in range(300)], np.int32)
+    fan = alls.MeshArrays(coords, conn, 2, 3, uniform_type=alls.T3)
+    meshes = [alls.generate(alls.GEN_T3, 12, 9, jitter=0.2, seed=31, random_diagonal=1), fan,
+              alls.generate(alls.GEN_T3, 7, 7, jitter=0.2, seed=32)]
+    out, row_off, nnz_off = ctx.assemble_batch(meshes)
+    got = out.download()
+    out.free()
+    check_batch(ctx, meshes, got, row_off, nnz_off)
+    compare_csr(alls.split_batch(got, row_off, nnz_off)[1], oracle.assemble(fan))

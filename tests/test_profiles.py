@@ -19,7 +19,7 @@ def test_stage_kernels_have_measured_traffic():
         kernels = json.load(f)["kernels"]
     for stage, name in bench.STAGE_KERNEL.items():
         assert name in kernels, f"{stage}: {name} missing from the traffic summary"
-        traffic, source = bench.measured_traffic(stage)
+        traffic, source = bench.measured_traffic(name)
         assert traffic and traffic > 0 and source
 
 
