@@ -269,9 +269,10 @@ void launch_jpl_round(const DevMesh& m, const int64_t* off, const int32_t* list,
                       int32_t* misc, hipStream_t s);
 void launch_conflicts(const DevMesh& m, const int64_t* off, const int32_t* list, const int32_t* col, int32_t* misc,
                       hipStream_t s);
-void launch_color_count(const int32_t* col, int64_t n, int32_t* cnt, hipStream_t s);
-void launch_color_bucket(const int32_t* col, int64_t n, const int64_t* coff, int32_t* cursor, int32_t* elist,
-                         hipStream_t s);
+// nc: number of colours (every col[e] lies in [0, nc)).
+void launch_color_count(const int32_t* col, int64_t n, int nc, int32_t* cnt, hipStream_t s);
+void launch_color_bucket(const int32_t* col, int64_t n, int nc, const int64_t* coff, int32_t* cursor,
+                         int32_t* elist, hipStream_t s);
 // One source mesh of a batch (device arrays) and its place in the packed mesh.
 struct PackMesh {
     const double* coords;

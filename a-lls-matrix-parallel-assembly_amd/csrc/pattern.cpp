@@ -98,7 +98,7 @@ alls_status bucket_colors(alls_coloring* c, const alls_pattern* p, alls_ctx* ctx
     HIP_TRY(c->elist.ensure(std::max<int64_t>(n, 1) * sizeof(int32_t)), "alloc colour lists");
     launch_fill_i32(cnt.as<int32_t>(), nc, 0, s);
     launch_fill_i32(cur.as<int32_t>(), nc, 0, s);
-    launch_color_count(c->colors.as<int32_t>(), n, cnt.as<int32_t>(), s);
+    launch_color_count(c->colors.as<int32_t>(), n, nc, cnt.as<int32_t>(), s);
     std::vector<int32_t> hc(nc);
     HIP_TRY(hipMemcpyAsync(hc.data(), cnt.p, nc * sizeof(int32_t), hipMemcpyDeviceToHost, s), "read colour counts");
     HIP_TRY(hipStreamSynchronize(s), "sync colour counts");
@@ -106,7 +106,8 @@ alls_status bucket_colors(alls_coloring* c, const alls_pattern* p, alls_ctx* ctx
     for (int32_t i = 0; i < nc; ++i) c->offsets[i + 1] = c->offsets[i] + hc[i];
     HIP_TRY(hipMemcpyAsync(coff.p, c->offsets.data(), (nc + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s),
             "write colour offsets");
-    launch_color_bucket(c->colors.as<int32_t>(), n, coff.as<int64_t>(), cur.as<int32_t>(), c->elist.as<int32_t>(), s);
+    launch_color_bucket(c->colors.as<int32_t>(), n, nc, coff.as<int64_t>(), cur.as<int32_t>(), c->elist.as<int32_t>(),
+                        s);
     HIP_TRY(hipStreamSynchronize(s), "sync colour buckets");
     cnt.release();
     cur.release();
