@@ -35,7 +35,7 @@ class Opts(C.Structure):
     _fields_ = [
         ("precision", C.c_int32), ("chunk_slots", C.c_int32), ("block_threads", C.c_int32),
         ("row_ptr_i64", C.c_int32), ("timing", C.c_int32), ("grid_per_cu", C.c_int32),
-        ("autotune", C.c_int32), ("reserved", C.c_int32),
+        ("autotune", C.c_int32), ("val_f32", C.c_int32),
     ]
 
 
@@ -43,7 +43,7 @@ class Csr(C.Structure):
     _fields_ = [
         ("n_rows", C.c_int64), ("nnz", C.c_int64), ("row_ptr_is_i64", C.c_int32), ("pattern_id", C.c_int32),
         ("row_ptr", C.c_void_p), ("col_ind", C.c_void_p), ("val", C.c_void_p),
-        ("cap_rows", C.c_int64), ("cap_nnz", C.c_int64),
+        ("cap_rows", C.c_int64), ("cap_nnz", C.c_int64), ("val_is_f32", C.c_int32), ("_pad", C.c_int32),
     ]
 
 
@@ -307,12 +307,13 @@ class Context:
         return s.as_dict()
 
     def assemble(self, mesh, precision=PREC_F64, chunk_slots=0, block_threads=0, row_ptr_i64=0,
-                 timing=1, out=None, c_mesh=None, grid_per_cu=0, autotune=0):
-        """Plain A-LLS assembly; returns a DeviceCsr (reused if `out` is given)."""
+                 timing=1, out=None, c_mesh=None, grid_per_cu=0, autotune=0, val_f32=0):
+        """Plain A-LLS assembly; returns a DeviceCsr (reused if `out` is given).
+        val_f32=1 (Hex20): float output values, downloaded widened to float64."""
         o = Opts()
         o.precision, o.chunk_slots, o.block_threads = precision, chunk_slots, block_threads
         o.row_ptr_i64, o.timing = row_ptr_i64, timing
-        o.grid_per_cu, o.autotune = grid_per_cu, autotune
+        o.grid_per_cu, o.autotune, o.val_f32 = grid_per_cu, autotune, val_f32
         out = out or DeviceCsr(self)
         m = c_mesh if c_mesh is not None else mesh.c_mesh()
         check(lib().alls_assemble(self.handle, C.byref(m), C.byref(o), C.byref(out.csr)), self)
@@ -320,18 +321,18 @@ class Context:
 
 
     def assemble_noalloc(self, dmesh, out, precision=PREC_F64, chunk_slots=0, block_threads=0, grid_per_cu=0,
-                         c_mesh=None):
+                         c_mesh=None, val_f32=0):
         """alls_assemble_noalloc: a DeviceMesh into the caller's CallerCsr
         buffers, with no device allocation by the library."""
         o = Opts()
         o.precision, o.chunk_slots, o.block_threads = precision, chunk_slots, block_threads
-        o.grid_per_cu, o.timing = grid_per_cu, 1
+        o.grid_per_cu, o.timing, o.val_f32 = grid_per_cu, 1, val_f32
         m = c_mesh if c_mesh is not None else dmesh.c_mesh()
         check(lib().alls_assemble_noalloc(self.handle, C.byref(m), C.byref(o), C.byref(out.csr)), self)
         return out
 
     def assemble_batch(self, meshes, precision=PREC_F64, chunk_slots=0, block_threads=0, grid_per_cu=0,
-                       autotune=0, out=None, c_meshes=None):
+                       autotune=0, out=None, c_meshes=None, val_f32=0):
         """alls_assemble_batch: one pipeline over all meshes (same dim).  Returns
         (DeviceCsr of the block-diagonal result, row_off, nnz_off), offsets as
         numpy int64 arrays of len(meshes)+1."""
@@ -339,7 +340,7 @@ class Context:
         arr = (Mesh * n)(*[(c_meshes[k] if c_meshes is not None else meshes[k].c_mesh()) for k in range(n)])
         o = Opts()
         o.precision, o.chunk_slots, o.block_threads = precision, chunk_slots, block_threads
-        o.grid_per_cu, o.autotune, o.timing = grid_per_cu, autotune, 1
+        o.grid_per_cu, o.autotune, o.timing, o.val_f32 = grid_per_cu, autotune, 1, val_f32
         out = out or DeviceCsr(self)
         row_off = np.zeros(n + 1, np.int64)
         nnz_off = np.zeros(n + 1, np.int64)
@@ -507,15 +508,15 @@ class CallerCsr:
     row_ptr of cap_rows entries (int64 if row_ptr_i64), col_ind and val of
     cap_nnz entries, allocated here with hipMalloc and freed with hipFree."""
 
-    def __init__(self, ctx, cap_rows, cap_nnz, row_ptr_i64=0):
+    def __init__(self, ctx, cap_rows, cap_nnz, row_ptr_i64=0, val_f32=0):
         self.ctx = ctx
         h = hip()
         self.csr = Csr()
         self.csr.row_ptr = h.malloc(cap_rows * (8 if row_ptr_i64 else 4))
         self.csr.col_ind = h.malloc(max(cap_nnz, 1) * 4)
-        self.csr.val = h.malloc(max(cap_nnz, 1) * 8)
+        self.csr.val = h.malloc(max(cap_nnz, 1) * (4 if val_f32 else 8))
         self.csr.cap_rows, self.csr.cap_nnz = cap_rows, cap_nnz
-        self.csr.row_ptr_is_i64 = row_ptr_i64
+        self.csr.row_ptr_is_i64, self.csr.val_is_f32 = row_ptr_i64, val_f32
 
     def download(self):
         return DeviceCsr.download(self)

@@ -310,8 +310,9 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
 // max_big: most entries in a row of the wave-per-row kernel (MISC_MAX_BIG).
 // big_list: the count pass's long-row list; when every row is long (3D
 // lists) the write pass reuses it for the rows it defers.
+// val: double[nnz], or float[nnz] when val_f32 (3D lists only; sums stay FP64).
 void launch_write(const ListPool& p, int small_group, int f32vals, int dpn, int32_t max_big, int32_t* big_list,
-                  const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s);
+                  const void* row_ptr, int i64, int32_t* col_ind, void* val, int val_f32, hipStream_t s);
 int64_t scan_tile_count(int64_t n);
 // Fused count + scan + write of 2D meshes whose rows all hold <= G entries
 // (tstat: fused_tile_count int64 tile states, zeroed; misc[MISC_TICKET] zero).

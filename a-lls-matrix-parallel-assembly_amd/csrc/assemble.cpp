@@ -421,6 +421,10 @@ alls_status classify_mesh(const alls_mesh* m, const alls_opts& o, MeshInfo* mi, 
         if (err) *err = "bad mem";
         return ALLS_E_INVALID;
     }
+    if (o.val_f32 && m->dim != 3) {
+        if (err) *err = "FP32 output values (opts.val_f32) are for Hex20 meshes";
+        return ALLS_E_UNSUPPORTED;
+    }
     if (m->dim == 3) {
         if (m->nodes_per_elem != 20 || (m->elem_type == nullptr && m->uniform_type != ALLS_HEX20)) {
             if (err) *err = "3D meshes must be Hex20 with nodes_per_elem = 20";
@@ -525,7 +529,7 @@ alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, De
     return ALLS_OK;
 }
 
-alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
+alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64, int f32) {
     if (out->cap_rows < n_rows + 1 || !out->row_ptr) {
         dev_free(out->row_ptr);
         out->row_ptr = nullptr;
@@ -537,7 +541,7 @@ alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
         out->cap_rows = n_rows + 1;
         out->pattern_id = 0;
     }
-    if (out->cap_nnz < nnz || !out->col_ind || !out->val) {
+    if (out->cap_nnz < nnz || !out->col_ind || !out->val || out->val_is_f32 != f32) {
         dev_free(out->col_ind);
         dev_free(out->val);
         out->col_ind = nullptr;
@@ -545,7 +549,8 @@ alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
         out->cap_nnz = 0;
         const int64_t n = std::max<int64_t>(nnz, 1);
         if (dev_alloc(reinterpret_cast<void**>(&out->col_ind), (size_t)n * sizeof(int32_t)) != hipSuccess ||
-            dev_alloc(reinterpret_cast<void**>(&out->val), (size_t)n * sizeof(double)) != hipSuccess) {
+            dev_alloc(reinterpret_cast<void**>(&out->val), (size_t)n * (f32 ? sizeof(float) : sizeof(double))) !=
+                hipSuccess) {
             return ALLS_E_OOM;
         }
         out->cap_nnz = n;
@@ -554,6 +559,7 @@ alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64) {
     out->n_rows = n_rows;
     out->nnz = nnz;
     out->row_ptr_is_i64 = i64;
+    out->val_is_f32 = f32;
     return ALLS_OK;
 }
 
@@ -795,7 +801,7 @@ static alls_status enqueue_back(Workspace& ws, const DevMesh& dm, const MeshInfo
     HIP_TRY(hipGetLastError(), "launch scan apply");
     if (events) HIP_TRY(hipEventRecord(ws.ev[7], s), "event");
     launch_write(pl.lp, pl.G, mi.f32 ? 1 : 0, mi.dpn, pl.max_big, ws.big.as<int32_t>(), out->row_ptr,
-                 out->row_ptr_is_i64, out->col_ind, out->val, s);
+                 out->row_ptr_is_i64, out->col_ind, out->val, out->val_is_f32, s);
     HIP_TRY(hipGetLastError(), "launch write");
     if (events) HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
     return ALLS_OK;
@@ -947,7 +953,7 @@ static alls_status run_graph(Workspace& ws, const DevMesh& dm, const MeshInfo& m
     for (;;) {
         alls_status rs = alloc_lists(ws, dm, mi, pl, err);
         if (rs != ALLS_OK) return rs;
-        rs = ensure_csr(out, n_rows, bound, i64);
+        rs = ensure_csr(out, n_rows, bound, i64, o.val_f32 ? 1 : 0);
         if (rs != ALLS_OK) {
             if (err) *err = "cannot allocate output CSR";
             return rs;
@@ -1061,6 +1067,10 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
         // The caller's output buffers: checked, never reallocated.
         out->n_rows = n_rows;
         out->nnz = nnz;
+        if (out->val_is_f32 != (o.val_f32 ? 1 : 0)) {
+            if (err) *err = "opts->val_f32 and out->val_is_f32 disagree";
+            return ALLS_E_INVALID;
+        }
         if (out->cap_rows < n_rows + 1 || out->cap_nnz < nnz || (!out->row_ptr_is_i64 && nnz > (int64_t)INT32_MAX)) {
             if (err)
                 *err = "output buffers too small: need " + std::to_string(n_rows + 1) + " row_ptr entries and " +
@@ -1069,7 +1079,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
         }
     } else {
         const int i64 = (o.row_ptr_i64 || nnz > (int64_t)INT32_MAX) ? 1 : 0;
-        rs = ensure_csr(out, n_rows, nnz, i64);
+        rs = ensure_csr(out, n_rows, nnz, i64, o.val_f32 ? 1 : 0);
         if (rs != ALLS_OK) {
             if (err) *err = "cannot allocate output CSR";
             return rs;
@@ -1280,8 +1290,15 @@ alls_status alls_csr_download(alls_ctx* ctx, const alls_csr* csr, int64_t* row_p
     if (col_ind && csr->col_ind && csr->nnz)
         HIP_TRY(hipMemcpy(col_ind, csr->col_ind, (size_t)csr->nnz * sizeof(int32_t), hipMemcpyDeviceToHost),
                 "download col_ind");
-    if (val && csr->val && csr->nnz)
-        HIP_TRY(hipMemcpy(val, csr->val, (size_t)csr->nnz * sizeof(double), hipMemcpyDeviceToHost), "download val");
+    if (val && csr->val && csr->nnz) {
+        if (csr->val_is_f32) {
+            std::vector<float> tmp((size_t)csr->nnz);
+            HIP_TRY(hipMemcpy(tmp.data(), csr->val, tmp.size() * sizeof(float), hipMemcpyDeviceToHost), "download val");
+            for (size_t i = 0; i < tmp.size(); ++i) val[i] = tmp[i];
+        } else {
+            HIP_TRY(hipMemcpy(val, csr->val, (size_t)csr->nnz * sizeof(double), hipMemcpyDeviceToHost), "download val");
+        }
+    }
     return ALLS_OK;
 }
 

@@ -136,6 +136,10 @@ alls_status alls_assemble_batch(alls_ctx* ctx, int32_t n, const alls_mesh* meshe
     (void)hipSetDevice(ctx->device);
     alls_opts o{};
     if (opts) o = *opts;
+    if (o.val_f32) {
+        ctx->err = "FP32 output values (opts.val_f32) are for alls_assemble and alls_assemble_noalloc";
+        return ALLS_E_UNSUPPORTED;
+    }
     alls_stats stats{};
     const alls_status st = batch_assemble(ctx->ws, ctx->stream, n, meshes, o, out, row_off, nnz_off, &stats,
                                           &ctx->err, ctx, /*use_graph=*/true);

@@ -255,7 +255,7 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
     const int gbase = wl - lane;
 #pragma unroll
     for (int b = 0; b < B; ++b) stage[threadIdx.x * B + b] = L.vh[b];
-    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1) << gbase);
+    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << (G & 63)) - 1) << gbase);
     uint64_t key = KEY_INF;
     int32_t slot = lane;
     if (lane < n) key = pack_key(kh);
@@ -318,9 +318,9 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
 
 // Head lanes write their dpn x dpn block at rank `rank` of the node's rows,
 // whose first entries are rs[dr].
-template <int B>
+template <int B, typename OT>
 __device__ __forceinline__ void small_row_store(int32_t col, int32_t rank, const double* v, const int64_t* rs,
-                                                int32_t* col_ind, double* val, int dpn) {
+                                                int32_t* col_ind, OT* val, int dpn) {
     if constexpr (B == 4) {
         // 2 DOF per node: row starts are even, so each block row is one
         // aligned int2 + double2 store.
@@ -335,7 +335,7 @@ __device__ __forceinline__ void small_row_store(int32_t col, int32_t rank, const
             const int64_t q = rs[dr] + (int64_t)rank * dpn;
             for (int dc = 0; dc < dpn; ++dc) {
                 col_ind[q + dc] = col * dpn + dc;
-                val[q + dc] = v[dr * dpn + dc];
+                val[q + dc] = (OT)v[dr * dpn + dc];
             }
         }
     }
@@ -345,9 +345,9 @@ __device__ __forceinline__ void small_row_store(int32_t col, int32_t rank, const
 // loads of all R rounds are issued before the first row is merged.  Row of
 // round r: wave base + r * 64/G + group, so that each round reads and writes
 // one contiguous stretch as a single round would.
-template <int G, int R, int NG, int B, typename VT, bool WRITE, typename RP>
+template <int G, int R, int NG, int B, typename VT, bool WRITE, typename RP, typename OT>
 __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int32_t* big_list,
-                                                    const RP* row_ptr, int32_t* col_ind, double* val,
+                                                    const RP* row_ptr, int32_t* col_ind, OT* val,
                                                     int dpn) {
     constexpr int GPW = 64 / G;   // row groups per wave
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             if (row.head) {
                 int64_t rs[3];
                 for (int dr = 0; dr < dpn; ++dr) rs[dr] = (int64_t)row_ptr[a * dpn + dr];
-                small_row_store<B>(row.col, row.rank, row.v, rs, col_ind, val, dpn);
+                small_row_store<B, OT>(row.col, row.rank, row.v, rs, col_ind, val, dpn);
             }
         }
     }
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, 
 #pragma unroll
             for (int dr = 0; dr < D; ++dr) row_ptr[a * D + dr] = (RP)rs[dr];
         }
-        if (r[it].head) small_row_store<B>(r[it].col, r[it].rank, r[it].v, rs, col_ind, val, D);
+        if (r[it].head) small_row_store<B, double>(r[it].col, r[it].rank, r[it].v, rs, col_ind, val, D);
     }
 }
 
@@ -654,9 +654,9 @@ __device__ __forceinline__ int32_t wave_excl_scan(int32_t x, int32_t* total) {
 // Rows beyond k_rows_merge's LDS (more than MERGE_CAP_MAX entries): one
 // 64-thread block (one wave) per row at a time, a bitonic sort of `cap` keys
 // staged in LDS.
-template <int B, typename VT, bool WRITE, typename RP>
+template <int B, typename VT, bool WRITE, typename RP, typename OT>
 __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list, int32_t* u,
-                                                 const RP* row_ptr, int32_t* col_ind, double* val,
+                                                 const RP* row_ptr, int32_t* col_ind, OT* val,
                                                  int dpn, int cap) {
     extern __shared__ uint64_t smem_big[];
     uint64_t* sk = smem_big;
@@ -728,7 +728,7 @@ __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list
                     const int64_t rs = (int64_t)row_ptr[a * dpn + dr] + (int64_t)rank * dpn;
                     for (int dc = 0; dc < dpn; ++dc) {
                         col_ind[rs + dc] = col * dpn + dc;
-                        val[rs + dc] = v[dr * dpn + dc];
+                        val[rs + dc] = (OT)v[dr * dpn + dc];
                     }
                 }
                 ++rank;
@@ -754,9 +754,9 @@ __device__ __forceinline__ uint32_t col_hash(int32_t col, int shift) {
 //   6. lane per column: insertion-sort its bucket by element id, sum the
 //      values in that order and write the dpn x dpn block; lanes take
 //      consecutive columns, so the stores are contiguous.
-template <int B, typename VT, typename RP>
+template <int B, typename VT, typename RP, typename OT>
 __global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* list, const RP* row_ptr,
-                                                   int32_t* col_ind, double* val, int cap) {
+                                                   int32_t* col_ind, OT* val, int cap) {
     constexpr int D = B == 4 ? 2 : 3;
     extern __shared__ int32_t smem_merge[];
     const int ts = 2 * cap;
@@ -885,7 +885,7 @@ __global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* li
 #pragma unroll
                 for (int dc = 0; dc < D; ++dc) {
                     col_ind[rs + dc] = col * D + dc;
-                    val[rs + dc] = v[dr * D + dc];
+                    val[rs + dc] = (OT)v[dr * D + dc];
                 }
             }
         }
@@ -900,9 +900,9 @@ __global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* li
 // element-id sort) run, and then hands them over through LDS, one DOF row
 // (dpn values per entry) at a time, in bucket order.  One global-memory
 // round trip per row instead of one per bucket member.
-template <int B, typename VT, typename RP, int EPL>
+template <int B, typename VT, typename RP, int EPL, typename OT>
 __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t* list, const RP* row_ptr,
-                                                       int32_t* col_ind, double* val, int count_slot) {
+                                                       int32_t* col_ind, OT* val, int count_slot) {
     constexpr int D = B == 4 ? 2 : 3;
     constexpr int CAP = 64 * EPL;
     constexpr int TS = 2 * CAP <= 128 ? 128 : 2 * CAP <= 256 ? 256 : 2 * CAP <= 512 ? 512 : 1024;   // power of two
@@ -1094,7 +1094,7 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
 #pragma unroll
                 for (int dc = 0; dc < D; ++dc) {
                     col_ind[rs + dc] = col * D + dc;
-                    val[rs + dc] = acc[dc];
+                    val[rs + dc] = (OT)acc[dc];
                 }
             }
             __syncthreads();
@@ -1135,9 +1135,9 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int l) {
 // segments of one element, more than UMAX columns) are listed in `defer`
 // (count misc[MISC_DEFER]) for k_rows_merge_reg.
 constexpr int SEG_UMAX = 96;
-template <typename VT, typename RP, int K, int SEG>
+template <typename VT, typename RP, int K, int SEG, typename OT>
 __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* row_ptr, int32_t* col_ind,
-                                                          double* val, int32_t* defer) {
+                                                          OT* val, int32_t* defer) {
     constexpr int B = 9, D = 3;
     constexpr int SPS = 64 / SEG;        // segments per register slot
     constexpr int LPS = SPS * SEG;       // lanes per slot
@@ -1334,7 +1334,7 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
 #pragma unroll
                 for (int dc = 0; dc < D; ++dc) {
                     col_ind[rs + dc] = col * D + dc;
-                    val[rs + dc] = w[dr * D + dc];
+                    val[rs + dc] = (OT)w[dr * D + dc];
                 }
             }
         }
@@ -1644,9 +1644,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply_vec(const int32_t* 
     }
 }
 
-template <int B, typename VT, bool WRITE, typename RP>
+template <int B, typename VT, bool WRITE, typename RP, typename OT = double>
 void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big_list, int32_t* u, int32_t* bl,
-                   const RP* rp, int32_t* ci, double* v, int dpn, hipStream_t s) {
+                   const RP* rp, int32_t* ci, OT* v, int dpn, hipStream_t s) {
     if (G > 0) {
         // Counting is bound by load latency: two rounds of rows per wave
         // keep more loads in flight (about 8% faster on Q4).  The write pass
@@ -1663,7 +1663,7 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
             }
         }
 #define ALLS_SMALL(GG, NGG) \
-        hipLaunchKernelGGL((k_rows_small<GG, R, NGG, B, VT, WRITE, RP>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn)
+        hipLaunchKernelGGL((k_rows_small<GG, R, NGG, B, VT, WRITE, RP, OT>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn)
         if constexpr (B == 4) {   // 2D: Q4 groups of 4; T3 and mixed keys
             if (G == 16) {
                 if (p.ng == 4) ALLS_SMALL(16, 4);
@@ -1725,42 +1725,42 @@ long_rows:
                 // 8 rows each, whose next row waits for the last one's stores.
                 grid = std::max<int64_t>(p.n_nodes, 1);
                 if (ks == 2)
-                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 2, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 2, 20, OT>), dim3((unsigned)grid), dim3(64), 0, s, p,
                                        rp, ci, v, bl);
                 else if (ks == 3)
-                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 3, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 3, 20, OT>), dim3((unsigned)grid), dim3(64), 0, s, p,
                                        rp, ci, v, bl);
                 else
-                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 4, 20>), dim3((unsigned)grid), dim3(64), 0, s, p,
+                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 4, 20, OT>), dim3((unsigned)grid), dim3(64), 0, s, p,
                                        rp, ci, v, bl);
-                hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4>), dim3(1024), dim3(64), 0, s, p, bl, rp, ci, v,
+                hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4, OT>), dim3(1024), dim3(64), 0, s, p, bl, rp, ci, v,
                                    MISC_DEFER);
                 return;
             }
         }
         if (epl == 1)
-            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 1>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 1, OT>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
                                rp, ci, v, MISC_BIG_COUNT);
         else if (epl == 2)
-            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 2>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 2, OT>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
                                rp, ci, v, MISC_BIG_COUNT);
         else if (epl == 3)
-            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 3>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 3, OT>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
                                rp, ci, v, MISC_BIG_COUNT);
         else
-            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
+            hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4, OT>), dim3((unsigned)grid), dim3(64), 0, s, p, bl_or_all,
                                rp, ci, v, MISC_BIG_COUNT);
         return;
     }
     if (cap <= MERGE_CAP_MAX) {
-        hipLaunchKernelGGL((k_rows_merge<B, VT, RP>), dim3((unsigned)grid), dim3(64), merge_lds(cap), s, p,
+        hipLaunchKernelGGL((k_rows_merge<B, VT, RP, OT>), dim3((unsigned)grid), dim3(64), merge_lds(cap), s, p,
                            bl_or_all, rp, ci, v, cap);
         return;
     }
     // Rows of more than MERGE_CAP_MAX entries (a node shared by very many
     // elements): sort staged in LDS, up to LONG_CAP_MAX entries.
     const size_t lds = (size_t)cap * (sizeof(uint64_t) + sizeof(uint16_t));
-    hipLaunchKernelGGL((k_rows_big<B, VT, WRITE, RP>), dim3((unsigned)grid), dim3(64), lds, s, p,
+    hipLaunchKernelGGL((k_rows_big<B, VT, WRITE, RP, OT>), dim3((unsigned)grid), dim3(64), lds, s, p,
                        G > 0 ? big_list : nullptr, u, rp, ci, v, dpn, cap);
 }
 
@@ -1802,11 +1802,11 @@ void launch_rows_fused(const ListPool& p, int G, int64_t* tstat, void* row_ptr, 
 
 void launch_count(const ListPool& p, int G, int f32vals, int32_t* u, int32_t* big_list, hipStream_t s) {
     if (p.B == 4)
-        rows_dispatch<4, double, false, int32_t>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 2, s);
+        rows_dispatch<4, double, false, int32_t, double>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 2, s);
     else if (f32vals)
-        rows_dispatch<9, float, false, int32_t>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
+        rows_dispatch<9, float, false, int32_t, double>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
     else
-        rows_dispatch<9, double, false, int32_t>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
+        rows_dispatch<9, double, false, int32_t, double>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 3, s);
 }
 
 void launch_scan_total(const int32_t* u, int64_t n, int64_t* tiles, int64_t ntiles, hipStream_t s) {
@@ -1844,20 +1844,28 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
 }
 
 void launch_write(const ListPool& p, int G, int f32vals, int dpn, int32_t max_big, int32_t* big_list,
-                  const void* row_ptr, int i64, int32_t* col_ind, double* val, hipStream_t s) {
+                  const void* row_ptr, int i64, int32_t* col_ind, void* val, int val_f32, hipStream_t s) {
     (void)dpn;
     const int32_t mb = max_big;
+    double* vd = static_cast<double*>(val);
+    float* vf = static_cast<float*>(val);
+#define ALLS_W3(VT, RP, rp)                                                                                      \
+    do {                                                                                                         \
+        if (val_f32) rows_dispatch<9, VT, true, RP, float>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, vf, 3, s); \
+        else rows_dispatch<9, VT, true, RP, double>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, vd, 3, s);        \
+    } while (0)
     if (i64) {
         const int64_t* rp = static_cast<const int64_t*>(row_ptr);
-        if (p.B == 4) rows_dispatch<4, double, true, int64_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 2, s);
-        else if (f32vals) rows_dispatch<9, float, true, int64_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
-        else rows_dispatch<9, double, true, int64_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
+        if (p.B == 4) rows_dispatch<4, double, true, int64_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, vd, 2, s);
+        else if (f32vals) ALLS_W3(float, int64_t, rp);
+        else ALLS_W3(double, int64_t, rp);
     } else {
         const int32_t* rp = static_cast<const int32_t*>(row_ptr);
-        if (p.B == 4) rows_dispatch<4, double, true, int32_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 2, s);
-        else if (f32vals) rows_dispatch<9, float, true, int32_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
-        else rows_dispatch<9, double, true, int32_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, val, 3, s);
+        if (p.B == 4) rows_dispatch<4, double, true, int32_t>(p, G, mb, big_list, nullptr, big_list, rp, col_ind, vd, 2, s);
+        else if (f32vals) ALLS_W3(float, int32_t, rp);
+        else ALLS_W3(double, int32_t, rp);
     }
+#undef ALLS_W3
 }
 
 }  // namespace alls

@@ -117,8 +117,8 @@ alls_status stage_mesh(Workspace& ws, const alls_mesh* m, const MeshInfo& mi, De
 // are read back once, after it.
 alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, alls_csr* out,
                      hipStream_t s, alls_stats* st, std::string* err, bool use_graph = false);
-// Makes sure `out` can hold n_rows rows and nnz entries.
-alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64);
+// Makes sure `out` can hold n_rows rows and nnz entries (float values when f32).
+alls_status ensure_csr(alls_csr* out, int64_t n_rows, int64_t nnz, int i64, int f32 = 0);
 void free_csr(alls_csr* out);
 alls_status hip_fail(hipError_t e, const char* what, std::string* err);
 // Pattern cache internals shared with the server (pattern.cpp).

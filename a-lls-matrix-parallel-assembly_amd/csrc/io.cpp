@@ -58,7 +58,7 @@ alls_status alls_csr_write_mtx(alls_ctx* ctx, const alls_csr* csr, const char* p
         return ALLS_E_INVALID;
     }
     std::fprintf(out.f, "%%%%MatrixMarket matrix coordinate real general\n");
-    std::fprintf(out.f, "%% global stiffness matrix (A-LLS assembly, FP64)\n");
+    std::fprintf(out.f, "%% global stiffness matrix (A-LLS assembly, %s values)\n", csr->val_is_f32 ? "FP32" : "FP64");
     std::fprintf(out.f, "%lld %lld %lld\n", (long long)n, (long long)n, (long long)nnz);
     char line[64];
     for (int64_t r = 0; r < n; ++r)

@@ -325,6 +325,10 @@ alls_status alls_pattern_create(alls_ctx* ctx, const alls_mesh* mesh, const alls
     (void)hipSetDevice(ctx->device);
     alls_opts o{};
     if (opts) o = *opts;
+    if (o.val_f32) {
+        ctx->err = "FP32 output values (opts.val_f32) are for alls_assemble and alls_assemble_noalloc";
+        return ALLS_E_UNSUPPORTED;
+    }
     alls_stats st{};
     alls_status rc = pattern_build(ctx->ws, mesh, o, ctx->stream, ctx->device, pat, out, &st, &ctx->err);
     if (rc == ALLS_OK) ctx->stats = st;

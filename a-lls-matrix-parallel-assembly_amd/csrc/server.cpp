@@ -585,6 +585,7 @@ alls_status alls_server_submit(alls_server* s, const alls_mesh* mesh, uint64_t k
     r->mesh = *mesh;
     r->key = key;
     if (opts) r->opts = *opts;
+    if (r->opts.val_f32) return ALLS_E_UNSUPPORTED;   // FP32 output values: alls_assemble only
     MeshInfo mi;
     std::string err;
     alls_status rc = classify_mesh(mesh, r->opts, &mi, &err);

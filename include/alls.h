@@ -87,7 +87,9 @@ typedef struct {
                                  over the elements; 0 => one thread (2D) or one wave (Hex20) per element */
     int32_t autotune;         /* 1 => take block_threads / chunk_slots / grid_per_cu from the context's
                                  autotuner (tuned on first use for the mesh's kind and size class) */
-    int32_t reserved;
+    int32_t val_f32;          /* 1 => Hex20 only: the output values are float (summed in FP64, stored
+                                 as FP32), as the reference's Hex20 path returns them
+                                 (Hex:260, Hex:601-608); alls_assemble and alls_assemble_noalloc only */
 } alls_opts;
 
 /* CSR triple in device memory.  Zero-initialise before first use. */
@@ -99,9 +101,11 @@ typedef struct {
                                  these buffers already hold (0 = none) */
     void*    row_ptr;         /* device [n_rows+1] */
     int32_t* col_ind;         /* device [nnz] */
-    double*  val;             /* device [nnz] */
+    double*  val;             /* device [nnz]; float [nnz] when val_is_f32 */
     int64_t cap_rows;         /* capacity of row_ptr in rows (library bookkeeping) */
     int64_t cap_nnz;          /* capacity of col_ind/val in entries */
+    int32_t val_is_f32;       /* val holds floats (opts.val_f32) */
+    int32_t _pad;
 } alls_csr;
 
 /* Per-call statistics of the last assembly on a context. */
@@ -168,8 +172,9 @@ alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
  * race.  Values are bitwise those of alls_assemble. */
 alls_status alls_assemble_noalloc(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts, alls_csr* out);
 
-/* Copies a CSR to host memory.  row_ptr is always returned as int64.  Any
- * of the three destinations may be NULL. */
+/* Copies a CSR to host memory.  row_ptr is always returned as int64 and val
+ * as double (float values widened exactly).  Any of the three destinations
+ * may be NULL. */
 alls_status alls_csr_download(alls_ctx* ctx, const alls_csr* csr, int64_t* row_ptr,
                               int32_t* col_ind, double* val);
 void        alls_csr_free(alls_ctx* ctx, alls_csr* csr);

@@ -42,7 +42,7 @@ def test_struct_layouts_match_the_c_header(tmp_path):
     structs = {"alls_mesh": alls.Mesh, "alls_opts": alls.Opts, "alls_csr": alls.Csr,
                "alls_stats": alls.Stats, "alls_gen_spec": alls.GenSpec, "alls_host_mesh": alls.HostMesh,
                "alls_server_config": alls.ServerConfig, "alls_request_stats": alls.RequestStats,
-               "alls_peaks": alls.Peaks}
+               "alls_peaks": alls.Peaks, "alls_server_counts": alls.ServerCounts}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "alls.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

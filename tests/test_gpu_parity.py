@@ -394,3 +394,35 @@ def test_matrix_market_file_round_trips_exactly(ctx, tmp_path):
     assert K.shape == (rp.size - 1, rp.size - 1)
     assert np.array_equal(K.indptr, rp) and np.array_equal(K.indices, ci)
     assert np.array_equal(K.data, v)
+
+
+@pytest.mark.parametrize("prec", [alls.PREC_F64, alls.PREC_F32_MFMA], ids=["f64_integration", "f32_mfma"])
+def test_hex20_fp32_output_values(ctx, prec):
+    # opts.val_f32: the reference's Hex20 path returns float values
+    # (Hex:260, Hex:601-608).  Sums stay FP64 and only the final store narrows, so the
+    # float output is exactly the FP64-output result of the same path rounded
+    # to float; against the FP64 oracle it stays within 1e-6 Frobenius.
+    m = alls.generate(alls.GEN_HEX20, 5, 4, 3, jitter=0.15, seed=14, E_sigma=0.5)
+    out = ctx.assemble(m, precision=prec)
+    rp64, ci64, v64 = out.download()
+    out.free()
+    out = ctx.assemble(m, precision=prec, val_f32=1)
+    assert out.csr.val_is_f32 == 1
+    rp, ci, v = out.download()
+    out.free()
+    assert np.array_equal(rp, rp64) and np.array_equal(ci, ci64)
+    assert np.array_equal(v, v64.astype(np.float32).astype(np.float64))
+    ref = oracle.assemble(m)[2]
+    fro = float(np.linalg.norm(v - ref) / np.linalg.norm(ref))
+    print(f"hex20 float output, precision {prec}: Frobenius {fro:.3e}")
+    assert fro <= 1e-6
+
+
+def test_fp32_output_values_are_for_hex20_only(ctx):
+    m = alls.generate(alls.GEN_Q4, 6, 5)
+    with pytest.raises(alls.AllsError) as ei:
+        ctx.assemble(m, val_f32=1)
+    assert ei.value.status == alls.E_UNSUPPORTED
+    with pytest.raises(alls.AllsError) as ei:
+        ctx.assemble_batch([alls.generate(alls.GEN_HEX20, 2, 2, 2)], val_f32=1)
+    assert ei.value.status == alls.E_UNSUPPORTED

@@ -82,3 +82,14 @@ def test_noalloc_refusals(ctx):
     assert same(out.download(), ref)
     out.free()
     dm.free()
+
+
+def test_noalloc_hex20_fp32_output_values(ctx):
+    m = alls.generate(alls.GEN_HEX20, 6, 5, 4, jitter=0.15, seed=9)
+    ref = ctx.assemble(m, precision=alls.PREC_F32_MFMA, val_f32=1).download()
+    out = alls.CallerCsr(ctx, ref[0].size, ref[1].size, val_f32=1)
+    dm = alls.DeviceMesh(m)
+    ctx.assemble_noalloc(dm, out, precision=alls.PREC_F32_MFMA, val_f32=1)
+    assert same(out.download(), ref)
+    dm.free()
+    out.free()
