@@ -83,13 +83,24 @@ struct ListPool {
     int32_t B;                 // values per entry
     uint32_t cmagic;           // chunk_magic(C): s / C as a multiply-high
     int32_t long_row;          // reservations ending past this many entries record MISC_MAX_ROW
-    int32_t ng;                // slots per group (4 for Q4; the group code also takes 3); 0: per-entry keys
+    int32_t ng;                // 4: Q4 groups (element id per group in geid); 3: T3 line groups
+                               // (below); 0: per-entry keys
     int32_t gpc;               // 2D: groups per chunk (C / ng)
     int32_t npe;               // 2D: conn stride
     int32_t _pad;
 };
 
 #define ALLS_MAX_CHUNK 16384
+
+// T3 line groups (ng == 3).  Every reservation of a triangle mesh is one
+// group of 3 slots (the element's row block: columns node 0, 1, 2), and a
+// chunk holds C = 3 * gpc slots.  Each group lives in ONE 128-byte line of
+// `vals`: the three 32-byte value blocks of its slots, then a key block of
+// int32 {element, column 0, column 1, column 2} and 16 unused bytes.  The
+// insertion writes whole lines (one per element row, no separate key
+// array); slot indices (ch * C + r) are the same as for the other layouts,
+// and list_val_slot / list_key map them to the line.
+constexpr int LINE3_SLOTS = 4;   // value-block slots per 128-byte line group
 
 // Multiplier for s / C without a divide: floor(s * m / 2^32) with
 // m = ceil(2^32 / C) equals floor(s / C) for s < 2^32 / C (>= 262144 slots
@@ -213,6 +224,25 @@ ALLS_DEV void list_store_keys(const ListPool& p, int32_t a, int32_t pos, int nn,
         const int64_t ch = list_chunk(p, a, s);
         if (ch >= 0) p.keys[ch * p.C + (s - chunk_of(p, s) * p.C)] = make_int2(nd[j], e);
     }
+}
+
+// Value-block index of slot idx (ch * C + r): its place in the line groups
+// for T3 lists, idx itself otherwise.
+ALLS_DEV int64_t list_val_slot(const ListPool& p, int64_t idx) {
+    if (p.ng == 3) {
+        const int64_t g = idx / 3;
+        return g * LINE3_SLOTS + (idx - g * 3);
+    }
+    return idx;
+}
+// (column, element) key of slot idx.
+ALLS_DEV int2 list_key(const ListPool& p, int64_t idx) {
+    if (p.ng == 3) {
+        const int64_t g = idx / 3;
+        const int32_t* kb = reinterpret_cast<const int32_t*>(static_cast<const char*>(p.vals) + g * 128 + 96);
+        return make_int2(kb[1 + (int)(idx - g * 3)], kb[0]);
+    }
+    return p.keys[idx];
 }
 
 // r / ng for a slot r inside a chunk (ng = 3 or 4).

@@ -627,6 +627,7 @@ struct Pipeline {
     ListPool lp{};
     int64_t total = 0;     // unique (row node, column node) pairs
     int32_t max_big = 0;   // longest row of the wave-per-row kernels
+    int64_t list_bytes = 0;   // headers, keys, group ids and values of this call's pool
 };
 
 // List pool for pl.C (pool_chunks).  The exact bound of overflow chunks
@@ -640,15 +641,28 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     const size_t vsz = mi.f32 ? sizeof(float) : sizeof(double);
     const int64_t n_nodes = dm.n_nodes;
     pl.cap_chunks = pool_chunks(n_nodes, mi.entries_bound, pl.C, mi.kind, pl.full_pool);
+    // Q4 lists carry one element id per group of 4 slots (geid); the
+    // per-entry keys are written only for the rows the wave-per-row kernels
+    // take.  T3 lists are line groups: each group of 3 slots in one 128-byte
+    // line of vals with its key block, and no key array.  Mixed and Hex20
+    // lists key every entry.  (T3 with element ids per group paid a
+    // scattered connectivity load per group in the fused pass, 1.34 ms
+    // against 1.04 ms keyed at 1500x1500 with permuted node ids; the line
+    // groups carry the columns themselves.)
+    const int ng = mi.kind == 0 ? 4 : mi.kind == 1 ? 3 : 0;
+    const size_t keys_bytes = ng == 3 ? 0 : (size_t)pl.cap_chunks * pl.C * sizeof(int2);
+    const size_t geid_bytes = ng == 4 ? (size_t)pl.cap_chunks * (pl.C / 4) * sizeof(int32_t) : 0;
+    const size_t vals_bytes = ng == 3 ? (size_t)pl.cap_chunks * (pl.C / 3) * 128
+                                      : (size_t)pl.cap_chunks * pl.C * B * vsz;
+    pl.list_bytes = (int64_t)((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t) + keys_bytes +
+                              geid_bytes + vals_bytes);
     if (ws.ext) {
         // Fixed scratch (alls_assemble_noalloc): every array is carved from
         // it, 256-byte aligned, and nothing is allocated.
-        const int ngx = mi.kind == 0 ? 4 : 0;
         const size_t nn1 = (size_t)std::max<int64_t>(n_nodes, 1);
-        const size_t need[] = {nn1 * ALLS_DIR * sizeof(int32_t), (size_t)pl.cap_chunks * pl.C * sizeof(int2),
-                               ngx ? (size_t)pl.cap_chunks * (pl.C / ngx) * sizeof(int32_t) : 0,
-                               (size_t)pl.cap_chunks * pl.C * B * vsz, MISC_SIZE * sizeof(int32_t),
-                               nn1 * sizeof(int32_t), nn1 * sizeof(int32_t), (size_t)(pl.ntiles + 1) * sizeof(int64_t)};
+        const size_t need[] = {nn1 * ALLS_DIR * sizeof(int32_t), keys_bytes, geid_bytes, vals_bytes,
+                               MISC_SIZE * sizeof(int32_t), nn1 * sizeof(int32_t), nn1 * sizeof(int32_t),
+                               (size_t)(pl.ntiles + 1) * sizeof(int64_t)};
         DevBuf* bufs[] = {&ws.hdr, &ws.keys, &ws.geid, &ws.vals, &ws.misc, &ws.u, &ws.big, &ws.tiles};
         size_t off = 0;
         for (size_t k = 0; k < sizeof(need) / sizeof(need[0]); ++k) off += (need[k] + 255) & ~size_t(255);
@@ -666,22 +680,15 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
         }
     }
     HIP_TRY(ws.hdr.ensure((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t)), "alloc list headers");
-    // Q4 lists carry one element id per group of 4 slots; the per-entry
-    // keys are written only for the rows the wave-per-row kernels take.
-    // T3, mixed and Hex20 lists key every entry (T3 with group ids paid a
-    // scattered connectivity load per group in the fused pass, 1.34 ms
-    // against 1.04 ms keyed at 1500x1500 with permuted node ids, for 0.19 ms
-    // more insertion).
-    const int ng = mi.kind == 0 ? 4 : 0;
-    HIP_TRY(ws.keys.ensure((size_t)pl.cap_chunks * pl.C * sizeof(int2)), "alloc list keys");
-    if (ng) HIP_TRY(ws.geid.ensure((size_t)pl.cap_chunks * (pl.C / ng) * sizeof(int32_t)), "alloc list groups");
-    HIP_TRY(ws.vals.ensure((size_t)pl.cap_chunks * pl.C * B * vsz), "alloc list values");
+    if (keys_bytes) HIP_TRY(ws.keys.ensure(keys_bytes), "alloc list keys");
+    if (geid_bytes) HIP_TRY(ws.geid.ensure(geid_bytes), "alloc list groups");
+    HIP_TRY(ws.vals.ensure(vals_bytes), "alloc list values");
     HIP_TRY(ws.misc.ensure(MISC_SIZE * sizeof(int32_t)), "alloc misc");
     HIP_TRY(ws.u.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc row counts");
     HIP_TRY(ws.big.ensure((size_t)std::max<int64_t>(n_nodes, 1) * sizeof(int32_t)), "alloc big-row list");
     HIP_TRY(ws.tiles.ensure((size_t)(pl.ntiles + 1) * sizeof(int64_t)), "alloc scan tiles");
     pl.lp.hdr = ws.hdr.as<int32_t>();
-    pl.lp.keys = ws.keys.as<int2>();
+    pl.lp.keys = keys_bytes ? ws.keys.as<int2>() : nullptr;
     pl.lp.vals = ws.vals.p;
     pl.lp.misc = ws.misc.as<int32_t>();
     pl.lp.n_nodes = n_nodes;
@@ -692,7 +699,7 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     pl.lp.long_row = pl.G > 0 ? pl.G : INT32_MAX;
     pl.lp.ng = ng;
     pl.lp.gpc = ng ? pl.C / ng : 0;
-    pl.lp.geid = ng ? ws.geid.as<int32_t>() : nullptr;
+    pl.lp.geid = geid_bytes ? ws.geid.as<int32_t>() : nullptr;
     pl.lp.conn = dm.conn;
     pl.lp.n_elems = dm.n_elems;
     pl.lp.npe = dm.npe;
@@ -1003,7 +1010,7 @@ static alls_status run_graph(Workspace& ws, const DevMesh& dm, const MeshInfo& m
         st->list_entries = mi.entries_bound;
         st->chunk_slots = pl.C;
         st->retries = pl.retries;
-        st->scratch_bytes = (int64_t)(ws.hdr.bytes + ws.keys.bytes + ws.geid.bytes + ws.vals.bytes);
+        st->scratch_bytes = pl.list_bytes;
         const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
         st->big_rows = f[MISC_BIG_COUNT];
         st->pool_chunks_used = dm.n_nodes + f[MISC_OVF_TOP];
@@ -1023,6 +1030,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     pl.C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
     pl.C = std::min(std::max(pl.C, 4), ALLS_MAX_CHUNK);
     if (mi.kind == 0) pl.C = std::max(4, pl.C / 4 * 4);   // whole slot groups per chunk
+    if (mi.kind == 1) pl.C = std::max(3, pl.C / 3 * 3);   // whole T3 line groups per chunk
     pl.G = small_group(mi.kind);
     pl.ntiles = scan_tile_count(n_nodes);
     if (use_graph) return run_graph(ws, dm, mi, o, pl, out, s, st, err);
@@ -1108,7 +1116,7 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
         st->list_entries = mi.entries_bound;
         st->chunk_slots = pl.C;
         st->retries = pl.retries;
-        st->scratch_bytes = (int64_t)(ws.hdr.bytes + ws.keys.bytes + ws.geid.bytes + ws.vals.bytes);
+        st->scratch_bytes = pl.list_bytes;
         const int32_t* f = reinterpret_cast<const int32_t*>(ws.host_misc);
         st->big_rows = f[MISC_BIG_COUNT];
         st->pool_chunks_used = n_nodes + f[MISC_OVF_TOP];

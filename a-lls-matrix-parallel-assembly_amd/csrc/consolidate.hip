@@ -99,10 +99,10 @@ __device__ __forceinline__ int64_t entry_index(const ListPool& p, int64_t a, con
     return ch * p.C + (s - k * p.C);
 }
 
-// Group layout, with the group size NG (3 or 4) known at compile time (NG = 0
-// means per-entry keys).  gkey: the key of slot r (any slot of the row with
-// the same r % NG) given its group's element e: the column node is
-// conn[e][r % NG].  An element id out of range (a group left unwritten after
+// List layout as a compile-time NG: 4 = Q4 groups (element id per group of 4
+// slots, in geid), 3 = T3 line groups (key block inside each group's 128-byte
+// line), 0 = per-entry keys.  gkey (Q4 groups): the key of slot r given its
+// group's element e: the column node is conn[e][r % 4].  An element id out of range (a group left unwritten after
 // a failed chunk wait, which the host then retries) reads as column a with
 // *pad set (its value counts as zero), so no access leaves the arrays.
 template <int NG>
@@ -194,14 +194,33 @@ struct SmallLoad {
     VT vh[B];
 };
 
+// T3 line groups: key and value-block index of in-chunk slot r of chunk ch.
+// The column and the element are two loads from computed addresses (no
+// select on loaded data), so that callers can issue the loads of several
+// rows before waiting for any.
+__device__ __forceinline__ int2 line3_key(const ListPool& p, int64_t ch, int32_t r, int64_t* vslot) {
+    const int32_t g = (int32_t)(__umulhi((uint32_t)r, 0xAAAAAAABu) >> 1), q = r - 3 * g;
+    const int64_t line = ch * p.gpc + g;
+    const int32_t* kb = reinterpret_cast<const int32_t*>(static_cast<const char*>(p.vals) + line * 128 + 96);
+    *vslot = line * LINE3_SLOTS + q;
+    return make_int2(kb[1 + q], kb[0]);
+}
+__device__ __forceinline__ int32_t line3_col(const ListPool& p, int64_t ch, int32_t r) {
+    const int32_t g = (int32_t)(__umulhi((uint32_t)r, 0xAAAAAAABu) >> 1), q = r - 3 * g;
+    const int64_t line = ch * p.gpc + g;
+    return reinterpret_cast<const int32_t*>(static_cast<const char*>(p.vals) + line * 128 + 96)[1 + q];
+}
+
 template <int NG, int B, typename VT, bool VALS>
 __device__ __forceinline__ void small_row_load(const ListPool& p, int64_t a, int lane, SmallLoad<B, VT>& L) {
     L.kh = make_int2(0, 0);
     if (lane < p.C) {
-        if (NG) L.kh.y = p.geid[a * p.gpc + grp<NG>(lane)];   // group layout: the group's element
-        else L.kh = p.keys[a * p.C + lane];
+        int64_t vs = a * p.C + lane;
+        if constexpr (NG == 4) L.kh.y = p.geid[a * p.gpc + grp<NG>(lane)];   // group layout: the group's element
+        else if constexpr (NG == 3) L.kh = line3_key(p, a, lane, &vs);
+        else L.kh = p.keys[vs];
         if constexpr (VALS) {
-            const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + lane) * B;
+            const VT* src = static_cast<const VT*>(p.vals) + vs * B;
 #pragma unroll
             for (int b = 0; b < B; ++b) L.vh[b] = src[b];
         }
@@ -223,14 +242,15 @@ __device__ __forceinline__ void small_row_keys(const ListPool& p, int64_t a, int
         int64_t ch;
         int32_t r;
         entry_loc(p, a, p.hdr + a * ALLS_DIR, lane, &ch, &r);
-        const int64_t ei = ch * p.C + r;
-        if (NG) L.kh.y = p.geid[ch * p.gpc + grp<NG>(r)];
+        int64_t ei = ch * p.C + r;
+        if constexpr (NG == 4) L.kh.y = p.geid[ch * p.gpc + grp<NG>(r)];
+        else if constexpr (NG == 3) L.kh = line3_key(p, ch, r, &ei);
         else L.kh = p.keys[ei];
         const VT* src = static_cast<const VT*>(p.vals) + ei * B;
 #pragma unroll
         for (int b = 0; b < B; ++b) L.vh[b] = src[b];
     }
-    if (NG && lane < n) {   // group layout: column from the group's element
+    if (NG == 4 && lane < n) {   // group layout: column from the group's element
         bool pad;
         L.kh = gkey<NG>(p, a, lane, L.kh.y, &pad);
         if (pad) {
@@ -364,8 +384,11 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             kc[r] = 0;
             hn[r] = 0;
             if (a < p.n_nodes) {
-                if (lane < p.C)   // with the header, not after it
-                    kc[r] = NG ? p.geid[a * p.gpc + grp<NG>(lane)] : p.keys[a * p.C + lane].x;
+                if (lane < p.C) {   // with the header, not after it
+                    if constexpr (NG == 4) kc[r] = p.geid[a * p.gpc + grp<NG>(lane)];
+                    else if constexpr (NG == 3) kc[r] = line3_col(p, a, lane);
+                    else kc[r] = p.keys[a * p.C + lane].x;
+                }
                 hn[r] = min(p.hdr[a * ALLS_DIR], ALLS_DIR * p.C);
             }
         }
@@ -379,9 +402,11 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
                 int64_t ch;
                 int32_t rr;
                 entry_loc(p, a, p.hdr + a * ALLS_DIR, lane, &ch, &rr);
-                kc[r] = NG ? p.geid[ch * p.gpc + grp<NG>(rr)] : p.keys[ch * p.C + rr].x;
+                if constexpr (NG == 4) kc[r] = p.geid[ch * p.gpc + grp<NG>(rr)];
+                else if constexpr (NG == 3) kc[r] = line3_col(p, ch, rr);
+                else kc[r] = p.keys[ch * p.C + rr].x;
             }
-            if (NG && lane < n) {
+            if (NG == 4 && lane < n) {
                 bool pad;
                 kc[r] = gkey<NG>(p, a, lane, kc[r], &pad).x;
             }
@@ -393,7 +418,7 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             const int32_t n = hn[r];
             if (n > G) {
                 if (lane == 0) big_list[atomicAdd(&p.misc[MISC_BIG_COUNT], 1)] = (int32_t)a;
-                if (NG) materialize_keys<NG, B, VT>(p, a, lane, G, n);
+                if (NG == 4) materialize_keys<NG, B, VT>(p, a, lane, G, n);
                 continue;
             }
             uint32_t c = lane < n ? (uint32_t)kc[r] : 0xffffffffu;
@@ -674,7 +699,7 @@ __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list
         int P = 1;
         while (P < n) P <<= 1;
         for (int i = lane; i < P; i += 64) {
-            sk[i] = i < n ? pack_key(p.keys[entry_index(p, a, h, i)]) : KEY_INF;
+            sk[i] = i < n ? pack_key(list_key(p, entry_index(p, a, h, i))) : KEY_INF;
             ss[i] = (uint16_t)i;
         }
         __syncthreads();
@@ -720,7 +745,7 @@ __global__ __launch_bounds__(64) void k_rows_big(ListPool p, const int32_t* list
 #pragma unroll
                 for (int b = 0; b < B; ++b) v[b] = 0.0;
                 for (int j = i; j < n && key_col(sk[j]) == col; ++j) {
-                    const VT* src = static_cast<const VT*>(p.vals) + entry_index(p, a, h, ss[j]) * B;
+                    const VT* src = static_cast<const VT*>(p.vals) + list_val_slot(p, entry_index(p, a, h, ss[j])) * B;
 #pragma unroll
                     for (int b = 0; b < B; ++b) v[b] += (double)src[b];
                 }
@@ -789,7 +814,7 @@ __global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* li
         __syncthreads();
         // 1. keys -> hash set; eu[i] = slot of the entry's column.
         for (int i = lane; i < n; i += 64) {
-            const int2 k = p.keys[entry_index(p, a, h, i)];
+            const int2 k = list_key(p, entry_index(p, a, h, i));
             eeid[i] = k.y;
             uint32_t slot = col_hash(k.x, shift);
             for (int probe = 0; probe < ts; ++probe) {
@@ -874,7 +899,7 @@ __global__ __launch_bounds__(64) void k_rows_merge(ListPool p, const int32_t* li
 #pragma unroll
             for (int b = 0; b < B; ++b) v[b] = 0.0;
             for (int m = g0; m < g1; ++m) {
-                const VT* src = static_cast<const VT*>(p.vals) + entry_index(p, a, h, memb[m]) * B;
+                const VT* src = static_cast<const VT*>(p.vals) + list_val_slot(p, entry_index(p, a, h, memb[m])) * B;
 #pragma unroll
                 for (int b = 0; b < B; ++b) v[b] += (double)src[b];
             }
@@ -939,8 +964,8 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
             for (int b = 0; b < B; ++b) vv[k][b] = VT(0);
             if (i < p.C) {
                 const int64_t ei = a * p.C + i;
-                kk[k] = p.keys[ei];
-                const VT* src = static_cast<const VT*>(p.vals) + ei * B;
+                kk[k] = list_key(p, ei);
+                const VT* src = static_cast<const VT*>(p.vals) + list_val_slot(p, ei) * B;
 #pragma unroll
                 for (int b = 0; b < B; ++b) vv[k][b] = src[b];
             }
@@ -957,8 +982,8 @@ __global__ __launch_bounds__(64) void k_rows_merge_reg(ListPool p, const int32_t
             const int i = lane + 64 * k;
             if (i >= p.C && i < n) {   // overflow chunks
                 const int64_t ei = dir.entry(p, a, i);
-                kk[k] = p.keys[ei];
-                const VT* src = static_cast<const VT*>(p.vals) + ei * B;
+                kk[k] = list_key(p, ei);
+                const VT* src = static_cast<const VT*>(p.vals) + list_val_slot(p, ei) * B;
 #pragma unroll
                 for (int b = 0; b < B; ++b) vv[k][b] = src[b];
             }
@@ -1365,8 +1390,8 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
         // The first two keys of this lane in the home chunk are loaded
         // together with the row header (they do not depend on it).
         int32_t hk0 = 0, hk1 = 0;
-        if (lane < p.C) hk0 = p.keys[a * p.C + lane].x;
-        if (lane + 64 < p.C) hk1 = p.keys[a * p.C + lane + 64].x;
+        if (lane < p.C) hk0 = list_key(p, a * p.C + lane).x;
+        if (lane + 64 < p.C) hk1 = list_key(p, a * p.C + lane + 64).x;
         RowDir dir;
         dir.load(p.hdr + a * ALLS_DIR);
         const int32_t n = min(dir.d[0], ALLS_DIR * p.C);
@@ -1383,7 +1408,7 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
         __syncthreads();
         int32_t mine = 0;
         for (int i = lane; i < n; i += 64) {
-            const int32_t col = i < p.C && i < lane + 128 ? (i == lane ? hk0 : hk1) : p.keys[dir.entry(p, a, i)].x;
+            const int32_t col = i < p.C && i < lane + 128 ? (i == lane ? hk0 : hk1) : list_key(p, dir.entry(p, a, i)).x;
             uint32_t slot = col_hash(col, shift);
             for (int probe = 0; probe < tsr; ++probe) {
                 const int32_t old = atomicCAS(&table[slot], -1, col);
@@ -1664,15 +1689,18 @@ void rows_dispatch(const ListPool& p, int G, int32_t max_big, const int32_t* big
         }
 #define ALLS_SMALL(GG, NGG) \
         hipLaunchKernelGGL((k_rows_small<GG, R, NGG, B, VT, WRITE, RP, OT>), dim3(blocks), dim3(256), 0, s, p, u, bl, rp, ci, v, dpn)
-        if constexpr (B == 4) {   // 2D: Q4 groups of 4; T3 and mixed keys
+        if constexpr (B == 4) {   // 2D: Q4 groups of 4, T3 line groups, mixed keys
             if (G == 16) {
                 if (p.ng == 4) ALLS_SMALL(16, 4);
+                else if (p.ng == 3) ALLS_SMALL(16, 3);
                 else ALLS_SMALL(16, 0);
             } else if (G == 32) {
                 if (p.ng == 4) ALLS_SMALL(32, 4);
+                else if (p.ng == 3) ALLS_SMALL(32, 3);
                 else ALLS_SMALL(32, 0);
             } else {
-                ALLS_SMALL(64, 0);
+                if (p.ng == 3) ALLS_SMALL(64, 3);
+                else ALLS_SMALL(64, 0);
             }
         } else {
             if (G == 16) ALLS_SMALL(16, 0);
@@ -1787,6 +1815,7 @@ void launch_rows_fused(const ListPool& p, int G, int64_t* tstat, void* row_ptr, 
 #define ALLS_FU_NG(GG, RP)                     \
     do {                                       \
         if (p.ng == 4) ALLS_FU(GG, 4, RP);     \
+        else if (p.ng == 3) ALLS_FU(GG, 3, RP); \
         else ALLS_FU(GG, 0, RP);               \
     } while (0)
     if (G == 16) {

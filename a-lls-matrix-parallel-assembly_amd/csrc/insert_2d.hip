@@ -28,7 +28,8 @@ typedef double f64x2 __attribute__((ext_vector_type(2)));
 // Rows are padded (9 x 16 B) so that the lane-strided staging writes spread
 // over the LDS banks.
 struct RowStage {
-    double2 v[64][9];      // entry j as (K00, K01), (K10, K11); [8] is padding
+    double2 v[64][9];      // entry j as (K00, K01), (K10, K11) (T3: [6], [7] hold the
+                           // line group's key block); [8] is padding
     int32_t row[64], pos[64], nn[64];
 };
 
@@ -75,12 +76,17 @@ __device__ __forceinline__ int32_t match_insert(MatchTable& t, int32_t key) {
 // segment), instead of every lane scattering its own 16-B pieces over 64
 // rows.  Scattered stores kept the address unit busy (rocprof TA_TA_BUSY ~80%
 // of the kernel); grouped stores touch about a sixth of the cache lines per
-// instruction.  For Q4 and T3 meshes the key is the group's element id (4 B
-// per element row, stored straight from the element's lane): the column of
+// instruction.  For Q4 meshes the key is the group's element id (4 B per
+// element row, stored straight from the element's lane): the column of
 // entry j is the element's node j, so the entries carry no key of their own.
 // Per-entry keys cost 0.27 ms of 0.85 at Q4 2000x2000 (4M extra cache lines
-// touched by partial writes); the element ids touch 0.5M.  Mixed meshes
-// (groups of 3 and 4) keep per-entry keys, stored by the element's lane.
+// touched by partial writes); the element ids touch 0.5M.  T3 meshes write
+// each element row as one whole 128-byte line (3 value blocks and a key
+// block; ListPool ng == 3): 96-byte groups and separate 24-byte keys split
+// into partial lines at random rows, which measured 0.91 ms for the stores
+// alone at 1500x1500 with permuted node ids against 0.38 ms as whole lines.
+// Mixed meshes (groups of 3 and 4) keep per-entry keys, stored by the
+// element's lane.
 //
 // Grid-stride over blocks of elements (persistent grid, or one thread per
 // element): `base` is uniform over the block, so every wave takes the same
@@ -172,8 +178,9 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
             __syncthreads();   // the row stage is refilled below
         }
         __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        // Triangles have 3 row nodes; the loop over a fourth is skipped.
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < (MODE == 1 ? 3 : 4); ++i) {
             const bool has = i < nn;
             st.nn[lane] = has ? nn : 0;
             if (has) {
@@ -188,10 +195,46 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                         st.v[lane][2 * j + 1] = make_double2(k[2], k[3]);
                     }
                 }
-                if (MODE == 0) list_store_group(p, el.nd[i], pos[i], (int32_t)e);
-                else list_store_keys(p, el.nd[i], pos[i], nn, el.nd, (int32_t)e);
+                if constexpr (MODE == 1) {
+                    // Key block of the line group: {element, node 0, node 1,
+                    // node 2} as raw bits, then 16 unused bytes (zero), so
+                    // that the 8 pieces cover the whole 128-byte line.
+                    st.v[lane][6] = make_double2(
+                        __longlong_as_double((int64_t)(((uint64_t)(uint32_t)el.nd[0] << 32) | (uint32_t)e)),
+                        __longlong_as_double((int64_t)(((uint64_t)(uint32_t)el.nd[2] << 32) | (uint32_t)el.nd[1])));
+                    st.v[lane][7] = make_double2(0.0, 0.0);
+                } else if constexpr (MODE == 0) {
+                    list_store_group(p, el.nd[i], pos[i], (int32_t)e);
+                } else {
+                    list_store_keys(p, el.nd[i], pos[i], nn, el.nd, (int32_t)e);
+                }
             }
             __syncthreads();
+            if constexpr (MODE == 1) {
+                // T3 line groups: 8 lanes write one element row's 128-byte
+                // line (3 value blocks + key block).  Every reservation is a
+                // whole group (3 slots from a multiple of 3), so a group
+                // never straddles chunks.
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int piece = t * 64 + lane;
+                    const int src = piece >> 3, w = piece & 7;
+                    if (st.nn[src] > 0) {
+                        const int32_t s = st.pos[src];
+                        const int64_t ch = list_chunk(p, st.row[src], s);
+                        if (ch >= 0) {
+                            const int64_t line = ch * p.gpc + (s - chunk_of(p, s) * p.C) / 3;
+                            // Non-temporal, as the other layouts' value
+                            // stores: each store instruction covers whole
+                            // lines (C3 T3 1500x1500: 0.77 -> 0.45 ms).
+                            const double2 x = st.v[src][w];
+                            __builtin_nontemporal_store(f64x2{x.x, x.y}, reinterpret_cast<f64x2*>(p.vals) + line * 8 + w);
+                        }
+                    }
+                }
+                __syncthreads();
+                continue;
+            }
 #pragma unroll
             for (int t = 0; t < 8; ++t) {   // values: lane -> (element src, entry j, half)
                 const int piece = t * 64 + lane;
@@ -204,11 +247,9 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                         // Non-temporal: the lists are read once, by another
                         // kernel (Q4 2000x2000: 0.58 -> 0.54 ms, and the
                         // count pass after it gained 6%; mixed 1500x1500:
-                        // 0.86 -> 0.84 ms).  Not for T3, whose 96-B groups
-                        // share cache lines: 0.78 -> 0.95 ms that way.
+                        // 0.86 -> 0.84 ms).
                         const double2 x = st.v[src][w];
-                        if (MODE == 1) reinterpret_cast<double2*>(p.vals)[idx * 2 + (w & 1)] = x;
-                        else __builtin_nontemporal_store(f64x2{x.x, x.y}, reinterpret_cast<f64x2*>(p.vals) + idx * 2 + (w & 1));
+                        __builtin_nontemporal_store(f64x2{x.x, x.y}, reinterpret_cast<f64x2*>(p.vals) + idx * 2 + (w & 1));
                     }
                 }
             }
