@@ -90,9 +90,9 @@ def intermediate_bytes(mesh, st, val_bytes, key_bytes):
     entries the insertion appends (values + keys or group ids) and the
     consolidation passes read back."""
     entries = st["list_entries"]
-    return {"insert": entries * (val_bytes + key_bytes),
-            "count": entries * key_bytes,
-            "write": entries * (val_bytes + key_bytes)}
+    return {"insert": int(entries * (val_bytes + key_bytes)),
+            "count": int(entries * key_bytes),
+            "write": int(entries * (val_bytes + key_bytes))}
 
 
 def hbm_roofline(comp_bytes, stage_ms, stage_kernel):
@@ -162,7 +162,7 @@ def footprint(mesh, st):
 
 # Kernel behind each pipeline stage of the Q4 workload (rocprofv3 names).
 STAGE_KERNEL = {"insert": "k_insert_2d<0>", "count": "k_count_quad<4, double>",
-                "write": "k_rows_small<16, 1, 4, 4, double, true, int>"}
+                "write": "k_rows_small<16, 1, 4, 4, double, true, int, double>"}
 
 
 def measured_traffic(kernel):
@@ -284,9 +284,10 @@ def hex20_compare(ctx, n, steps, peaks):
         times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
         t = "double" if name == "f64" else "float"
         kernels = {"insert": f"k_hex20<{t}, 0>", "count": "k_rows_count_long<4>",
-                   "write": f"k_rows_merge_seg<{t}, int, 3, 20>"}
+                   "write": f"k_rows_merge_seg<{t}, int, 3, 20, double>"}
         pb = path_bytes(m, st["nnz"])
         res[name]["roofline"] = hbm_roofline(pb, times, kernels)
+        res[name]["roofline"]["traffic"] = measured_traffic(res[name]["roofline"]["kernel"])[0]
         res[name]["path"] = path_block(pb, ms)
         peak = FP64_PEAK_TFLOPS if name == "f64" else FP32_PEAK_TFLOPS
         meas = peaks["mfma_f64_TFLOPs"] if name == "f64" else peaks["mfma_f32_TFLOPs"]
@@ -351,12 +352,15 @@ def c3_compare(ctx, n, steps, peaks):
              "elements_per_s": round(m.n_elems * steps / dt, 1), "ms_insert": round(st["ms_insert"], 3),
              "ms_count": round(st["ms_count"], 3), "ms_write": round(st["ms_write"], 3), "big_rows": st["big_rows"]}
         times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
-        mode = 1 if name == "t3" else 2
-        kernels = {"insert": f"k_insert_2d<{mode}>", "write": "k_rows_fused<32, 0, int>"}
+        # T3 lists are line groups (3 value blocks + a 32-byte key block per
+        # 128-byte line, NG 3); mixed lists key every entry (8 bytes).
+        mode, ng = (1, 3) if name == "t3" else (2, 0)
+        kernels = {"insert": f"k_insert_2d<{mode}>", "write": f"k_rows_fused<32, {ng}, int>"}
         pb = path_bytes(m, st["nnz"])
         r["roofline"] = hbm_roofline(pb, times, kernels)
+        r["roofline"]["traffic"] = measured_traffic(r["roofline"]["kernel"])[0]
         r["path"] = path_block(pb, ms)
-        r["intermediate_bytes"] = intermediate_bytes(m, st, 4 * 8, 8)
+        r["intermediate_bytes"] = intermediate_bytes(m, st, 4 * 8, 32 / 3 if ng == 3 else 8)
         r["atomics"] = atomic_block(reservations(m), st["ms_insert"], peaks["atomic_f64_Gops"],
                                     merged_reservations(m))
         r["hbm_footprint"] = footprint(m, st)
