@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -628,7 +629,18 @@ struct Pipeline {
     int64_t total = 0;     // unique (row node, column node) pairs
     int32_t max_big = 0;   // longest row of the wave-per-row kernels
     int64_t list_bytes = 0;   // headers, keys, group ids and values of this call's pool
+    int ng = 0;               // list layout (ListPool::ng), fixed for the call
 };
+
+// T3 line groups sort 32-bit keys (column << 5 | element-order slot), so
+// they take meshes whose node ids fit 27 bits; larger triangle meshes key
+// every entry.  ALLS_T3_KEYED=1 in the environment forces the keyed layout
+// (a diagnostic switch: the tests run both layouts).
+static bool t3_line_groups(int64_t n_nodes) {
+    if (n_nodes >= (int64_t(1) << 27)) return false;
+    const char* f = std::getenv("ALLS_T3_KEYED");
+    return !(f && f[0] && std::strcmp(f, "0") != 0);
+}
 
 // List pool for pl.C (pool_chunks).  The exact bound of overflow chunks
 // (entries / C) is rarely approached, since most rows fit their home chunk:
@@ -649,7 +661,7 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     // scattered connectivity load per group in the fused pass, 1.34 ms
     // against 1.04 ms keyed at 1500x1500 with permuted node ids; the line
     // groups carry the columns themselves.)
-    const int ng = mi.kind == 0 ? 4 : mi.kind == 1 ? 3 : 0;
+    const int ng = pl.ng;
     const size_t keys_bytes = ng == 3 ? 0 : (size_t)pl.cap_chunks * pl.C * sizeof(int2);
     const size_t geid_bytes = ng == 4 ? (size_t)pl.cap_chunks * (pl.C / 4) * sizeof(int32_t) : 0;
     const size_t vals_bytes = ng == 3 ? (size_t)pl.cap_chunks * (pl.C / 3) * 128
@@ -1033,15 +1045,18 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     if (mi.kind == 1) pl.C = std::max(3, pl.C / 3 * 3);   // whole T3 line groups per chunk
     pl.G = small_group(mi.kind);
     pl.ntiles = scan_tile_count(n_nodes);
+    pl.ng = mi.kind == 0 ? 4 : (mi.kind == 1 && t3_line_groups(n_nodes)) ? 3 : 0;
     if (use_graph) return run_graph(ws, dm, mi, o, pl, out, s, st, err);
     const bool stage_events = true;
-    // T3 and mixed meshes (32-lane rows) whose rows all fit the register
-    // kernel take the fused pass (count + scan + write in one launch), into an
-    // output sized by the entries bound; if a row turns out longer, the
-    // separate passes run on the same lists.  Measured on MI355X, the fused
-    // pass saves 13% of the T3/mixed path, while for Q4 (16-lane rows, a
-    // count pass of 0.32 ms at C2) the look-back costs more than it saves.
-    bool try_fused = !ws.ext && pl.G == 32 && n_nodes > 0;
+    // T3 line groups whose rows all fit the register kernel take the fused
+    // pass (count + scan + write in one launch), into an output sized by the
+    // entries bound; if a row turns out longer, the separate passes run on
+    // the same lists.  A count pass over line groups touches every line of a
+    // row for its columns (C3 T3 1500x1500: count 0.40 + write 0.65 ms against
+    // 1.00 ms fused).  Keyed lists (mixed meshes) count from their compact
+    // keys and take the separate passes (C3 mixed: count 0.23 + write 0.62
+    // against 1.00 ms fused); so does Q4, whose quad count takes 0.09 ms.
+    bool try_fused = !ws.ext && pl.ng == 3 && pl.G == 32 && n_nodes > 0;
     bool fused_done = false;
     for (;;) {
         alls_status rs = alloc_lists(ws, dm, mi, pl, err);

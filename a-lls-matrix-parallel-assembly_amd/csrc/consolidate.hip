@@ -336,6 +336,80 @@ __device__ __forceinline__ bool small_row_merge(const ListPool& p, int64_t a, in
     return true;
 }
 
+// small_row_merge for T3 line groups in 32-lane rows: the same sums with a
+// 32-bit sort key.  Every group of 3 slots is one element row (slots 3g ..
+// 3g+2, columns node 0, 1, 2 of the element), so (column, element, slot)
+// order is (column, rank of the group by (element, group), slot in group):
+// key = column << 5 | (group rank * 3 + slot in group), for columns below
+// 2^27 (the host keys every entry otherwise).  The values are parked in LDS
+// at that element-order slot, which the low 5 bits of a sorted key name.
+// The group ranks take one pass over the row's (at most 10) groups; the
+// 64-bit compare-exchange of (key, slot) pairs it replaces was most of the
+// pass's VALU work.
+template <int G, int B, typename VT>
+__device__ __forceinline__ bool small_row_merge_line3(int lane, SmallLoad<B, VT>& L, SmallRow<G, B, VT>& r,
+                                                      VT* stage) {
+    static_assert(G == 32, "line groups take 32-lane rows");
+    const int32_t n = L.n;
+    if (n > G) return false;
+    const int wl = threadIdx.x & 63;
+    const int gbase = wl - lane;
+    const bool valid = lane < n;
+    const int32_t g = (lane * 11) >> 5;   // lane / 3 for lanes 0..31
+    const int32_t q = lane - 3 * g;
+    const uint32_t e = (uint32_t)L.kh.y;
+    const int32_t ngr = (n + 2) / 3;
+    int32_t grank = 0;
+#pragma unroll
+    for (int gg = 0; gg < (G + 2) / 3; ++gg) {
+        if (!__any(gg < ngr)) break;
+        const uint32_t eo = (uint32_t)__shfl((int)e, gg * 3, G);
+        grank += (gg < ngr && (eo < e || (eo == e && gg < g))) ? 1 : 0;
+    }
+    const int32_t es = min(grank * 3 + q, G - 1);   // < n for every valid lane of a whole-group row
+    uint32_t key = valid ? (((uint32_t)L.kh.x << 5) | (uint32_t)es) : 0xffffffffu;
+    VT* gstage = stage + (threadIdx.x - lane) * B;   // the group's values, by element-order slot
+    if (valid) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) gstage[es * B + b] = L.vh[b];
+    }
+    // Distinct keys (valid lanes) sort first; the unused lanes' all-ones
+    // keys stay behind them.  Compare-exchange as min / max and a select
+    // (a conditional on the lane's direction compiles to divergent branches).
+#pragma unroll
+    for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t ok = xor_u32(key, j, G);
+            const bool keep_low = ((lane & k) == 0) == ((lane & j) == 0);
+            key = keep_low ? min(key, ok) : max(key, ok);
+        }
+    }
+    const uint64_t gmask = ((1ull << G) - 1) << gbase;
+    r.col = (int32_t)(key >> 5);
+    const uint32_t pk = up1_u32<G>(key);
+    r.head = valid && (lane == 0 || (pk >> 5) != (key >> 5));
+    const uint64_t gheads = __ballot(r.head) & gmask;
+    r.u = __popcll(gheads);
+    const uint64_t le = wl == 63 ? ~0ull : ((1ull << (wl + 1)) - 1);
+    r.rank = __popcll(gheads & le) - 1;
+    const uint64_t above = wl == 63 ? 0ull : (gheads & (~0ull << (wl + 1)));
+    const int32_t run = r.head ? (above ? (__ffsll((unsigned long long)above) - 1 - gbase) : n) - lane : 0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) r.v[b] = 0.0;
+    auto step = [&](int32_t sl) {
+#pragma unroll
+        for (int b = 0; b < B; ++b) r.v[b] += (double)gstage[sl * B + b];
+    };
+    if (run > 0) step((int32_t)(key & 31));
+    for (int d = 1; d < G; ++d) {
+        if (!__any(d < run)) break;
+        const int32_t sd = (int32_t)(down_u32<G>(key, d) & 31);
+        if (d < run) step(sd);
+    }
+    return true;
+}
+
 // Head lanes write their dpn x dpn block at rank `rank` of the node's rows,
 // whose first entries are rs[dr].
 template <int B, typename OT>
@@ -427,9 +501,8 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
 #pragma unroll
                 for (int j = k >> 1; j > 0; j >>= 1) {
                     const uint32_t oc = xor_u32(c, j, G);
-                    const bool up = (lane & k) == 0;
-                    const bool lower = (lane & j) == 0;
-                    if ((lower == up) ? (oc < c) : (oc > c)) c = oc;
+                    const bool keep_low = ((lane & k) == 0) == ((lane & j) == 0);
+                    c = keep_low ? min(c, oc) : max(c, oc);
                 }
             }
             const uint32_t pc = up1_u32<G>(c);
@@ -452,7 +525,10 @@ __global__ __launch_bounds__(256) void k_rows_small(ListPool p, int32_t* u, int3
             const int64_t a = wbase + r * GPW;
             if (a >= p.n_nodes) break;
             SmallRow<G, B, VT> row;
-            if (!small_row_merge<G, B, VT>(p, a, lane, L[r], row, vstage)) continue;   // long row: other kernel
+            bool fits;
+            if constexpr (NG == 3 && G == 32) fits = small_row_merge_line3<G, B, VT>(lane, L[r], row, vstage);
+            else fits = small_row_merge<G, B, VT>(p, a, lane, L[r], row, vstage);
+            if (!fits) continue;   // long row: other kernel
             if (row.head) {
                 int64_t rs[3];
                 for (int dr = 0; dr < dpn; ++dr) rs[dr] = (int64_t)row_ptr[a * dpn + dr];
@@ -585,7 +661,12 @@ __global__ __launch_bounds__(256) void k_rows_fused(ListPool p, int64_t* tstat, 
         const int64_t a = tile * RPT + it * RPI + rib;
         r[it].u = 0;
         r[it].head = false;
-        if (a < p.n_nodes && !small_row_merge<G, B, double>(p, a, lane, L[it], r[it], vstage)) {
+        bool fits = true;
+        if (a < p.n_nodes) {
+            if constexpr (NG == 3 && G == 32) fits = small_row_merge_line3<G, B, double>(lane, L[it], r[it], vstage);
+            else fits = small_row_merge<G, B, double>(p, a, lane, L[it], r[it], vstage);
+        }
+        if (!fits) {
             // the host checked that no row is longer than G; never taken
             if (lane == 0) atomicMax(&p.misc[MISC_ERR + ERR_ROW_TOO_BIG], G + 1);
             r[it].u = 0;

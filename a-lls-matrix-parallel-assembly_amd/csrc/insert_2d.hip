@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                         st.v[lane][2 * j + 1] = make_double2(k[2], k[3]);
                     }
                 }
-                if constexpr (MODE == 1) {
+                if (MODE == 1 && p.ng == 3) {
                     // Key block of the line group: {element, node 0, node 1,
                     // node 2} as raw bits, then 16 unused bytes (zero), so
                     // that the 8 pieces cover the whole 128-byte line.
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void k_insert_2d(DevMesh m, ListPool p) {
                 }
             }
             __syncthreads();
-            if constexpr (MODE == 1) {
+            if (MODE == 1 && p.ng == 3) {
                 // T3 line groups: 8 lanes write one element row's 128-byte
                 // line (3 value blocks + key block).  Every reservation is a
                 // whole group (3 slots from a multiple of 3), so a group

@@ -354,8 +354,14 @@ def c3_compare(ctx, n, steps, peaks):
         times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
         # T3 lists are line groups (3 value blocks + a 32-byte key block per
         # 128-byte line, NG 3); mixed lists key every entry (8 bytes).
-        mode, ng = (1, 3) if name == "t3" else (2, 0)
-        kernels = {"insert": f"k_insert_2d<{mode}>", "write": f"k_rows_fused<32, {ng}, int>"}
+        # T3 takes the fused pass; mixed meshes count and write separately.
+        if name == "t3":
+            mode, ng = 1, 3
+            kernels = {"insert": "k_insert_2d<1>", "write": "k_rows_fused<32, 3, int>"}
+        else:
+            mode, ng = 2, 0
+            kernels = {"insert": "k_insert_2d<2>", "count": "k_rows_small<32, 2, 0, 4, double, false, int, double>",
+                       "write": "k_rows_small<32, 1, 0, 4, double, true, int, double>"}
         pb = path_bytes(m, st["nnz"])
         r["roofline"] = hbm_roofline(pb, times, kernels)
         r["roofline"]["traffic"] = measured_traffic(r["roofline"]["kernel"])[0]

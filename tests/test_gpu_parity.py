@@ -426,3 +426,47 @@ def test_fp32_output_values_are_for_hex20_only(ctx):
     with pytest.raises(alls.AllsError) as ei:
         ctx.assemble_batch([alls.generate(alls.GEN_HEX20, 2, 2, 2)], val_f32=1)
     assert ei.value.status == alls.E_UNSUPPORTED
+
+
+T3_LAYOUT_CASES = [
+    ("fused", lambda: alls.generate(alls.GEN_T3, 60, 50, jitter=0.3, seed=21, random_diagonal=1, permute_nodes=1,
+                                    E_sigma=0.5), {}),
+    ("overflow_chunks", lambda: alls.generate(alls.GEN_T3, 30, 20, jitter=0.3, seed=22, random_diagonal=1,
+                                              permute_nodes=1), {"chunk_slots": 6}),
+    ("fan_40", lambda: fan_mesh(40), {}),
+    ("fan_300", lambda: fan_mesh(300), {}),
+    ("fan_700", lambda: fan_mesh(700), {}),
+    ("repeated_node", lambda: repeated_node_mesh(alls.GEN_T3, REPEATED[1][1])[1], {}),
+]
+
+
+@pytest.mark.parametrize("name,make,opts", T3_LAYOUT_CASES, ids=[c[0] for c in T3_LAYOUT_CASES])
+def test_t3_line_groups_equal_keyed_lists(ctx, monkeypatch, name, make, opts):
+    # Triangle meshes list each element row as one 128-byte line group and
+    # merge rows with 32-bit keys; meshes with node ids past 27 bits key every
+    # entry instead.  ALLS_T3_KEYED forces the keyed layout: both must give
+    # the same bits (the serial element order) and match the oracle, through
+    # the fused pass, overflow chunks, and the long-row kernels (fans).
+    m = make()
+    print(f"t3 layout case {name}", flush=True)
+    line = ctx.assemble(m, **opts).download()
+    monkeypatch.setenv("ALLS_T3_KEYED", "1")
+    keyed = ctx.assemble(m, **opts).download()
+    monkeypatch.delenv("ALLS_T3_KEYED")
+    assert same_csr(line, keyed)
+    compare_csr(line, oracle.assemble(m))
+
+
+def test_t3_batch_line_groups_equal_keyed_lists(ctx, monkeypatch):
+    meshes = [alls.generate(alls.GEN_T3, 12 + 3 * i, 9 + i, jitter=0.25, seed=40 + i, random_diagonal=1,
+                            permute_nodes=1, E_sigma=0.5) for i in range(5)]
+    out, ro, no = ctx.assemble_batch(meshes)
+    line = out.download()
+    out.free()
+    monkeypatch.setenv("ALLS_T3_KEYED", "1")
+    out, ro2, no2 = ctx.assemble_batch(meshes)
+    keyed = out.download()
+    out.free()
+    monkeypatch.delenv("ALLS_T3_KEYED")
+    assert np.array_equal(ro, ro2) and np.array_equal(no, no2)
+    assert same_csr(line, keyed)
