@@ -308,6 +308,19 @@ def hex20_compare(ctx, n, steps, peaks):
         dt_h, out = time_calls(lambda o: ctx.assemble(m, precision=prec, out=o), 2, 1)
         res[name]["host_input_ms"] = round(dt_h * 1e3, 3)
         out.free()
+    # The reference's Hex20 path returns float values: FP32-MFMA integration
+    # with FP32 output values (sums still FP64, only the stores narrow).
+    dt, st, out = time_steps(ctx, dm, steps, 1, precision=alls.PREC_F32_MFMA, val_f32=1)
+    ms = dt / steps * 1e3
+    r32 = {"ms_per_step": round(ms, 3), "elements_per_s": round(m.n_elems * steps / dt, 1),
+           "ms_insert": round(st["ms_insert"], 3), "ms_count": round(st["ms_count"], 3),
+           "ms_write": round(st["ms_write"], 3), "speedup_vs_f64": round(res["f64"]["ms_per_step"] / ms, 3)}
+    if v["f64"] is not None:
+        vf = out.download()[2]
+        r32["frobenius_vs_f64"] = float(np.linalg.norm(vf - v["f64"]) / np.linalg.norm(v["f64"]))
+        del vf
+    out.free()
+    res["f32_mfma_f32_values"] = r32
     # Cached pattern: value-only re-assembly (staged, no atomics) and the
     # coloured path with the 8-colour cell-parity colouring.
     for name, prec in (("f64", alls.PREC_F64), ("f32_mfma", alls.PREC_F32_MFMA)):
