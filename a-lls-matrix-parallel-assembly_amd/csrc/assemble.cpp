@@ -632,6 +632,20 @@ struct Pipeline {
     int ng = 0;               // list layout (ListPool::ng), fixed for the call
 };
 
+static bool t3_line_groups(int64_t n_nodes);
+
+// Chunk size, small-row lanes, scan tiles and list layout of a call.
+static void init_pipeline(Pipeline& pl, const alls_opts& o, const MeshInfo& mi, int64_t n_nodes) {
+    pl = Pipeline{};
+    pl.C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
+    pl.C = std::min(std::max(pl.C, 4), ALLS_MAX_CHUNK);
+    if (mi.kind == 0) pl.C = std::max(4, pl.C / 4 * 4);   // whole slot groups per chunk
+    if (mi.kind == 1) pl.C = std::max(3, pl.C / 3 * 3);   // whole T3 line groups per chunk
+    pl.G = small_group(mi.kind);
+    pl.ntiles = scan_tile_count(n_nodes);
+    pl.ng = mi.kind == 0 ? 4 : (mi.kind == 1 && t3_line_groups(n_nodes)) ? 3 : 0;
+}
+
 // T3 line groups sort 32-bit keys (column << 5 | element-order slot), so
 // they take meshes whose node ids fit 27 bits; larger triangle meshes key
 // every entry.  ALLS_T3_KEYED=1 in the environment forces the keyed layout
@@ -844,13 +858,17 @@ static alls_status run_fused(Workspace& ws, const DevMesh& dm, const MeshInfo& m
     }
     const int64_t nt = fused_tile_count(dm.n_nodes, pl.G);
     HIP_TRY(ws.tiles.ensure((size_t)std::max<int64_t>(nt, pl.ntiles + 1) * sizeof(int64_t)), "alloc tile states");
-    rs = enqueue_insert(ws, dm, mi, o, pl, s, true, err);
+    const bool events = o.timing != 0;
+    if (!events) HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
+    rs = enqueue_insert(ws, dm, mi, o, pl, s, events, err);
     if (rs != ALLS_OK) return rs;
     launch_fill_i32(ws.tiles.as<int32_t>(), nt * 2, 0, s);
-    HIP_TRY(hipEventRecord(ws.ev[3], s), "event");
-    HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
-    HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
-    HIP_TRY(hipEventRecord(ws.ev[7], s), "event");
+    if (events) {
+        HIP_TRY(hipEventRecord(ws.ev[3], s), "event");
+        HIP_TRY(hipEventRecord(ws.ev[4], s), "event");
+        HIP_TRY(hipEventRecord(ws.ev[5], s), "event");
+        HIP_TRY(hipEventRecord(ws.ev[7], s), "event");
+    }
     launch_rows_fused(pl.lp, pl.G, ws.tiles.as<int64_t>(), out->row_ptr, i64, out->col_ind, out->val, s);
     HIP_TRY(hipGetLastError(), "launch fused pass");
     HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
@@ -952,47 +970,68 @@ static alls_status graph_launch(GraphSlot& g, uint64_t key, hipStream_t s, F&& e
     return ALLS_OK;
 }
 
-// The whole pipeline as one captured graph (batch path).  The output is
-// sized by the entries bound (each list entry is at most one new column), so
-// no host read-back splits the pipeline, and the write pass's long-row
-// kernels are captured for rows of up to `cap` entries: the list directory's
-// bound in 2D (at most 256), three 60-entry segment slots for Hex20 (a
-// conforming brick's rows hold 160).  After the graph, one read-back of the
-// flags and the total: a list overflow rebuilds the lists and captures again;
-// a row longer than the captured kernels take (flagged, or the count pass's
-// longest row above `cap`) is written again, uncaptured, with the real bound.
-static alls_status run_graph(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, Pipeline& pl,
-                             alls_csr* out, hipStream_t s, alls_stats* st, std::string* err) {
+// The whole pipeline with ONE host synchronisation, at the end: the output
+// is sized by the entries bound (each list entry is at most one new column),
+// so no read-back of the total splits the pipeline, and the write pass's
+// long-row kernels are launched for rows of up to `cap` entries: the list
+// directory's bound in 2D (at most 256), three 60-entry segment slots for
+// Hex20 (a conforming brick's rows hold 160).  After it, one read-back of the
+// flags and the total: a list overflow rebuilds the lists and runs again; a
+// row longer than those kernels take (flagged, or the count pass's longest
+// row above `cap`) is written again with the real bound.  capture: the
+// pipeline is one hipGraph (batch path), captured on first use and replayed
+// while its arguments are unchanged; otherwise it is launched as is, with the
+// stage events when opts.timing is set.  *oom: the bound-sized output could
+// not be allocated (the caller falls back to an output of the exact size).
+// Plain 2D and Hex20 assemblies run this way: the second synchronisation and
+// read-back of the two-phase pipeline cost about 40 us of the 1.41 ms C2 step.
+static alls_status run_single(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const alls_opts& o, Pipeline& pl,
+                              alls_csr* out, hipStream_t s, alls_stats* st, std::string* err, bool capture,
+                              bool* oom) {
     const int dpn = mi.dpn;
     const int64_t n_rows = dm.n_nodes * dpn;
     const int64_t bound = std::max<int64_t>(mi.entries_bound, 1) * dpn * dpn;
     const int i64 = (o.row_ptr_i64 || bound > (int64_t)INT32_MAX) ? 1 : 0;
+    const bool events = !capture && o.timing;
     int launches = 0;
     int32_t cap = 0;
+    if (oom) *oom = false;
     for (;;) {
         alls_status rs = alloc_lists(ws, dm, mi, pl, err);
         if (rs != ALLS_OK) return rs;
         rs = ensure_csr(out, n_rows, bound, i64, o.val_f32 ? 1 : 0);
         if (rs != ALLS_OK) {
+            if (oom && rs == ALLS_E_OOM && !capture) {
+                *oom = true;
+                return ALLS_OK;
+            }
             if (err) *err = "cannot allocate output CSR";
             return rs;
         }
         cap = mi.kind == 3 ? 180 : std::min(ALLS_DIR * pl.C, 256);
         pl.max_big = cap;
-        HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
-        rs = graph_launch(ws.graph, mix(front_key(ws, dm, mi, o, pl), back_key(ws, dm, mi, pl, out)), s,
-                          [&] {
-                              alls_status r = enqueue_insert(ws, dm, mi, o, pl, s, false, err);
-                              if (r == ALLS_OK) r = enqueue_count(ws, dm, mi, pl, s, false, err, false);
-                              if (r == ALLS_OK) r = enqueue_back(ws, dm, mi, pl, out, s, false, err);
-                              if (r == ALLS_OK) r = enqueue_readback(ws, pl, s, err);
-                              return r;
-                          },
-                          err);
-        if (rs != ALLS_OK) return rs;
-        ++launches;
-        HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
-        HIP_TRY(hipStreamSynchronize(s), "sync after graph");
+        if (!events) HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
+        if (capture) {
+            rs = graph_launch(ws.graph, mix(front_key(ws, dm, mi, o, pl), back_key(ws, dm, mi, pl, out)), s,
+                              [&] {
+                                  alls_status r = enqueue_insert(ws, dm, mi, o, pl, s, false, err);
+                                  if (r == ALLS_OK) r = enqueue_count(ws, dm, mi, pl, s, false, err, false);
+                                  if (r == ALLS_OK) r = enqueue_back(ws, dm, mi, pl, out, s, false, err);
+                                  if (r == ALLS_OK) r = enqueue_readback(ws, pl, s, err);
+                                  return r;
+                              },
+                              err);
+            if (rs != ALLS_OK) return rs;
+            ++launches;
+        } else {
+            rs = enqueue_insert(ws, dm, mi, o, pl, s, events, err);
+            if (rs == ALLS_OK) rs = enqueue_count(ws, dm, mi, pl, s, events, err, false);
+            if (rs == ALLS_OK) rs = enqueue_back(ws, dm, mi, pl, out, s, events, err);
+            if (rs == ALLS_OK) rs = enqueue_readback(ws, pl, s, err);
+            if (rs != ALLS_OK) return rs;
+        }
+        if (!events) HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+        HIP_TRY(hipStreamSynchronize(s), capture ? "sync after graph" : "sync after pipeline");
         int32_t* f = reinterpret_cast<int32_t*>(ws.host_misc);
         const int32_t too_big = f[MISC_ERR + ERR_ROW_TOO_BIG];
         f[MISC_ERR + ERR_ROW_TOO_BIG] = 0;   // a write-pass flag here, not a list overflow
@@ -1001,10 +1040,10 @@ static alls_status run_graph(Workspace& ws, const DevMesh& dm, const MeshInfo& m
         if (rs != ALLS_OK) return rs;
         if (retry) continue;
         if (too_big || pl.max_big > cap) {
-            // Rows the captured write kernels could not take: write again.
-            rs = enqueue_back(ws, dm, mi, pl, out, s, false, err);
+            // Rows the write kernels launched could not take: write again.
+            rs = enqueue_back(ws, dm, mi, pl, out, s, events, err);
             if (rs != ALLS_OK) return rs;
-            HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
+            if (!events) HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
             HIP_TRY(hipStreamSynchronize(s), "sync after write");
         }
         break;
@@ -1015,7 +1054,15 @@ static alls_status run_graph(Workspace& ws, const DevMesh& dm, const MeshInfo& m
     if (st) {
         *st = alls_stats{};
         st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);
-        st->ms_write = st->ms_total;
+        if (events) {
+            st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
+            st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
+            st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
+            st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]) + ev_ms(ws.ev[5], ws.ev[7]);
+            st->ms_write = ev_ms(ws.ev[7], ws.ev[6]);
+        } else if (capture) {
+            st->ms_write = st->ms_total;
+        }
         st->n_elems = dm.n_elems;
         st->n_rows = n_rows;
         st->nnz = out->nnz;
@@ -1039,15 +1086,9 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     const int64_t n_nodes = dm.n_nodes;
     const int64_t n_rows = n_nodes * dpn;
     Pipeline pl;
-    pl.C = o.chunk_slots > 0 ? o.chunk_slots : default_chunk(mi.kind);
-    pl.C = std::min(std::max(pl.C, 4), ALLS_MAX_CHUNK);
-    if (mi.kind == 0) pl.C = std::max(4, pl.C / 4 * 4);   // whole slot groups per chunk
-    if (mi.kind == 1) pl.C = std::max(3, pl.C / 3 * 3);   // whole T3 line groups per chunk
-    pl.G = small_group(mi.kind);
-    pl.ntiles = scan_tile_count(n_nodes);
-    pl.ng = mi.kind == 0 ? 4 : (mi.kind == 1 && t3_line_groups(n_nodes)) ? 3 : 0;
-    if (use_graph) return run_graph(ws, dm, mi, o, pl, out, s, st, err);
-    const bool stage_events = true;
+    init_pipeline(pl, o, mi, n_nodes);
+    if (use_graph) return run_single(ws, dm, mi, o, pl, out, s, st, err, true, nullptr);
+    const bool stage_events = o.timing != 0;
     // T3 line groups whose rows all fit the register kernel take the fused
     // pass (count + scan + write in one launch), into an output sized by the
     // entries bound; if a row turns out longer, the separate passes run on
@@ -1057,7 +1098,19 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     // keys and take the separate passes (C3 mixed: count 0.23 + write 0.62
     // against 1.00 ms fused); so does Q4, whose quad count takes 0.09 ms.
     bool try_fused = !ws.ext && pl.ng == 3 && pl.G == 32 && n_nodes > 0;
+    // row_ptr is int32 unless nnz reaches 2^31 (or int64 is asked for), so an
+    // entries bound past 2^31 takes the two-phase pipeline, sized exactly.
+    const bool bound_fits = o.row_ptr_i64 || std::max<int64_t>(mi.entries_bound, 1) * dpn * dpn <= (int64_t)INT32_MAX;
+    if (!ws.ext && !try_fused && bound_fits) {
+        bool oom = false;
+        const alls_status rs1 = run_single(ws, dm, mi, o, pl, out, s, st, err, false, &oom);
+        if (rs1 != ALLS_OK || !oom) return rs1;
+        // No room for the bound-sized output: the two-phase pipeline below
+        // sizes it exactly.
+        init_pipeline(pl, o, mi, n_nodes);
+    }
     bool fused_done = false;
+    if (!stage_events) HIP_TRY(hipEventRecord(ws.ev[0], s), "event");
     for (;;) {
         alls_status rs = alloc_lists(ws, dm, mi, pl, err);
         if (rs != ALLS_OK) return rs;
@@ -1113,17 +1166,20 @@ alls_status run_alls(Workspace& ws, const DevMesh& dm, const MeshInfo& mi, const
     } else {
         rs = enqueue_back(ws, dm, mi, pl, out, s, stage_events, err);
         if (rs != ALLS_OK) return rs;
+        if (!stage_events) HIP_TRY(hipEventRecord(ws.ev[6], s), "event");
     }
     out->pattern_id = 0;
     HIP_TRY(hipStreamSynchronize(s), "sync after write");
     if (st) {
         *st = alls_stats{};
-        st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
-        st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
-        st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
-        // The scan's totals, then its application to row_ptr (ahead of the write).
-        st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]) + ev_ms(ws.ev[5], ws.ev[7]);
-        st->ms_write = ev_ms(ws.ev[7], ws.ev[6]);   // merge/write kernels alone
+        if (stage_events) {
+            st->ms_init = ev_ms(ws.ev[0], ws.ev[1]);
+            st->ms_insert = ev_ms(ws.ev[1], ws.ev[2]);
+            st->ms_count = ev_ms(ws.ev[2], ws.ev[3]);
+            // The scan's totals, then its application to row_ptr (ahead of the write).
+            st->ms_scan = ev_ms(ws.ev[3], ws.ev[4]) + ev_ms(ws.ev[5], ws.ev[7]);
+            st->ms_write = ev_ms(ws.ev[7], ws.ev[6]);   // merge/write kernels alone
+        }
         st->ms_total = ev_ms(ws.ev[0], ws.ev[6]);   // includes the flag read-back between the halves
         st->n_elems = dm.n_elems;
         st->n_rows = n_rows;

@@ -65,6 +65,7 @@ static alls_status tune(alls_ctx* ctx, const DevMesh& dm, const MeshInfo& mi, co
                 o.grid_per_cu = g;
                 o.chunk_slots = c;
                 o.autotune = 0;
+                o.timing = 1;   // candidates are ranked by their device time
                 double ms = 1e300;
                 for (int rep = 0; rep < 2; ++rep) {
                     alls_stats st{};

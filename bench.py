@@ -50,26 +50,37 @@ def c3_workload(name, n, seed=0xA11500C3):
     return alls.generate(kind, n, n, jitter=0.3, seed=seed, random_diagonal=1, permute_nodes=1, **kw)
 
 
-def time_steps(ctx, dmesh, steps, warmup, **opts):
-    """W untimed + K timed assemblies; returns (seconds, mean stats, out)."""
+def time_steps(ctx, dmesh, steps, warmup, stage_steps=5, **opts):
+    """W untimed + K timed assemblies, then a few instrumented ones; returns
+    (seconds of the K, their stats with stage times averaged over the
+    instrumented runs, out).  The timed runs ask for no stage timing
+    (opts.timing 0): the library then records no hipEvents between the
+    stages, which would add a few microseconds of idle time at each.  The
+    instrumented runs are the same pipeline with the events."""
     import alls
     cm = dmesh.c_mesh()
     out = None
     for _ in range(warmup):
-        out = ctx.assemble(dmesh.mesh, out=out, c_mesh=cm, **opts)
+        out = ctx.assemble(dmesh.mesh, out=out, c_mesh=cm, timing=0, **opts)
     alls.hip().sync()
-    acc = {}
     t0 = time.perf_counter()
     for _ in range(steps):
-        out = ctx.assemble(dmesh.mesh, out=out, c_mesh=cm, **opts)
+        out = ctx.assemble(dmesh.mesh, out=out, c_mesh=cm, timing=0, **opts)
+    alls.hip().sync()
+    dt = time.perf_counter() - t0
+    acc = {}
+    n_inst = max(1, min(steps, stage_steps))
+    for _ in range(n_inst):
+        out = ctx.assemble(dmesh.mesh, out=out, c_mesh=cm, timing=1, **opts)
         st = ctx.stats()
         for k in ("ms_total", "ms_init", "ms_insert", "ms_count", "ms_scan", "ms_write"):
             acc[k] = acc.get(k, 0.0) + st[k]
-    alls.hip().sync()
-    dt = time.perf_counter() - t0
     st = ctx.stats()
     for k in acc:
-        st[k] = acc[k] / steps
+        st[k] = acc[k] / n_inst
+    c = out.csr
+    st["out_allocated_bytes"] = (c.cap_rows * (8 if c.row_ptr_is_i64 else 4)
+                                 + c.cap_nnz * (4 + (4 if c.val_is_f32 else 8)))
     return dt, st, out
 
 
@@ -152,12 +163,16 @@ def atomic_block(n_res, insert_ms, atomic_peak_gops, issued=None):
 
 
 def footprint(mesh, st):
-    """HBM held by one assembly: inputs, list headers + pool, output CSR."""
+    """HBM held by one assembly: inputs, list headers + pool, output CSR (its
+    nnz, and the buffers allocated for it: room for the list-entries bound,
+    so that the pipeline needs one host synchronisation)."""
     inputs = path_bytes(mesh, 0) - 4 * (mesh.n_dofs + 1)
     rp = (mesh.n_dofs + 1) * (8 if st["nnz"] >= 2 ** 31 else 4)
     out = rp + 12 * st["nnz"]
+    alloc = st.get("out_allocated_bytes", out)
     lists = st["scratch_bytes"]
-    return {"inputs": inputs, "lists": lists, "output": out, "total": inputs + lists + out}
+    return {"inputs": inputs, "lists": lists, "output": out, "output_allocated": alloc,
+            "total": inputs + lists + alloc}
 
 
 # Kernel behind each pipeline stage of the Q4 workload (rocprofv3 names).

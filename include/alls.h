@@ -82,7 +82,8 @@ typedef struct {
     int32_t chunk_slots;      /* entries per list-pool chunk; 0 => tuned default; at most 16384 */
     int32_t block_threads;    /* threads per block of the insert kernel; 0 => tuned default */
     int32_t row_ptr_i64;      /* 1 => always int64 row_ptr; otherwise int64 only when nnz >= 2^31 */
-    int32_t timing;           /* 1 => time each stage with hipEvents (alls_last_stats) */
+    int32_t timing;           /* 1 => time each stage with hipEvents (alls_last_stats); 0 => the
+                                 pipeline records no events between its stages */
     int32_t grid_per_cu;      /* insert kernel grid: > 0 => persistent, this many blocks per CU looping
                                  over the elements; 0 => one thread (2D) or one wave (Hex20) per element */
     int32_t autotune;         /* 1 => take block_threads / chunk_slots / grid_per_cu from the context's
@@ -151,7 +152,13 @@ int         alls_device_count(void);
  *   assembleGlobalStiffnessKernelALLS   Acc:394-468 / T3:227-290 / Mix:248-312 (+ FixedArray, Hex:208-257)
  *   convertALLStoCSRKernel              Acc:471-501 / T3:292-314 / Mix:314-339 / Hex:259-278
  * and returns the values the reference's serial assembleGlobalStiffnessHost
- * (Acc:329-380) computes on that pattern. */
+ * (Acc:329-380) computes on that pattern.
+ * The output is allocated (or `out`'s buffers reused when large enough) with
+ * room for the list-entries bound, nnz <= entries x dpn^2 (cap_nnz), so that
+ * the pipeline runs with one host synchronisation; when that bound passes
+ * 2^31 (and opts->row_ptr_i64 is 0), or its allocation fails, the output is
+ * sized exactly after a read-back of nnz.  opts->timing 0 records no stage
+ * events (alls_last_stats then reports ms_total only). */
 alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts* opts, alls_csr* out);
 
 /* alls_assemble without any device allocation.  No reference counterpart:
