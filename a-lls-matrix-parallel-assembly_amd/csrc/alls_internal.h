@@ -152,6 +152,23 @@ ALLS_DEV int64_t xcd_block(int64_t bid, int64_t nb) {
     return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Grid-stride loop over [0, count) that keeps xcd_block's property: block
+// bid (XCD bid & 7, the (bid >> 3)-th block there) visits first, first +
+// step, ... below end, all inside its XCD's contiguous 1/8 of the range, so
+// a persistent grid's XCDs each sweep one slab of elements as a grid of one
+// block per element does.  Grids of fewer than 8 blocks stride plainly.
+struct XcdRange {
+    int64_t first, end, step;
+};
+ALLS_DEV XcdRange xcd_range(int64_t bid, int64_t nb, int64_t count) {
+    if (nb < 8) return XcdRange{bid, count, nb};
+    const int64_t x = bid & 7, i = bid >> 3;
+    const int64_t nbx = (nb >> 3) + (x < (nb & 7) ? 1 : 0);
+    const int64_t cq = count >> 3, cr = count & 7;
+    const int64_t lo = x * cq + (x < cr ? x : cr);
+    return XcdRange{lo + i, lo + cq + (x < cr ? 1 : 0), nbx};
+}
+
 // Reserve n consecutive slots of node-row a; returns the first slot.
 ALLS_DEV int32_t chunk_of(const ListPool& p, int32_t s) { return (int32_t)__umulhi((uint32_t)s, p.cmagic); }
 

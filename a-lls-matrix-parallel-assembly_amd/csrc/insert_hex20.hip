@@ -9,11 +9,12 @@
 //   M[(I,p),(J,q)] = sum_g w_g * dN_I/dx_p(g) * dN_J/dx_q(g),
 // the same sum the reference writes out term by term (Hex:173-191).  M is
 // the 60x60 product X^T (W X) with X the 27x60 matrix of Gauss-point
-// gradients, so one wavefront per element forms X in LDS and contracts it
-// on the matrix cores: v_mfma_f64_16x16x4f64 (FP64 path) or
-// v_mfma_f32_16x16x4f32 (FP32 path), over the 10 upper-triangle 16x16 tiles
-// of the symmetric 64x64 (padded) product, 7 k-steps of 4 Gauss points.
-// K is then formed from M in LDS and every row node's 20 entries are
+// gradients, so one wavefront per element forms X in LDS (54 lanes: a Gauss
+// point and half of the nodes each) and contracts it on the matrix cores:
+// v_mfma_f64_16x16x4f64 (FP64 path) or v_mfma_f32_16x16x4f32 (FP32 path),
+// over the 10 upper-triangle 16x16 tiles of the symmetric 64x64 (padded)
+// product, 7 k-steps of 4 Gauss points.  M is kept in LDS as its 210 upper
+// 3x3 node blocks, K is formed from them and every row node's 20 entries are
 // appended to its list with one reservation atomic.
 #include "alls_internal.h"
 
@@ -56,20 +57,31 @@ __device__ __forceinline__ void put_block_pair(double* dst, const double (&kb)[3
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Gauss rule exactly as the reference writes it (Hex:45-46).
-__constant__ double HGP[3] = {-0.774596669241483, 0.0, 0.774596669241483};
-__constant__ double HGW[3] = {0.555555555555556, 0.888888888888889, 0.555555555555556};
+constexpr double HGP[3] = {-0.774596669241483, 0.0, 0.774596669241483};
+constexpr double HGW[3] = {0.555555555555556, 0.888888888888889, 0.555555555555556};
 // Natural coordinates of the 20 nodes (Hex:97-99).
-__constant__ double HXI[20] = {-1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1, -1};
-__constant__ double HETA[20] = {-1, -1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1};
-__constant__ double HZETA[20] = {-1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1, 0, 0, 0, 0};
+constexpr double HXI[20] = {-1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1, -1};
+constexpr double HETA[20] = {-1, -1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 0, -1, -1, 1, 1};
+constexpr double HZETA[20] = {-1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1, 0, 0, 0, 0};
 
-constexpr int KS = 28;   // Gauss rows padded 27 -> 28 (7 k-steps of 4)
-constexpr int XS = 64;   // gradient columns padded 60 -> 64
-constexpr int MS = 61;   // row stride of M in LDS
+// LDS of one element (one wave), two phases over the same words:
+//   gradients, transposed and scaled: Xt[c * XT + g] = sqrt|w_g| *
+//   dN_{c/3}/dx_{c%3} at Gauss point g (c < 60; rows 60..63 are read by the
+//   padded MFMA tiles and only reach discarded outputs; the k-padding row
+//   g = 27 is masked at the load);
+//   M as its 210 upper 3x3 node blocks, pair (I <= J) at 9 * pair_index(I, J),
+//   row-major inside the block.
+constexpr int XT = 29;                  // Gauss points 27, padded to 28 k-rows, +1 against bank conflicts
+constexpr int NPAIR = 210;              // node pairs I <= J
+constexpr int HEX20_STAGE_ROWS_F32 = 3;   // node rows of list values staged per pass
+constexpr int HEX20_STAGE_ROWS_F64 = 2;
+constexpr int LDS_ELEMS = NPAIR * 9 > 64 * XT ? NPAIR * 9 : 64 * XT;
+
+// Index of node pair (I, J), I <= J, in the packed upper triangle.
+__device__ __forceinline__ int pair_index(int I, int J) { return I * (39 - I) / 2 + J; }
 
 // dN/d(xi,eta,zeta) of node n at a Gauss point (Hex:95-128).
-__device__ __forceinline__ void hex20_dshape(int n, double xi, double eta, double zeta, double& dx, double& de,
-                                             double& dz) {
+constexpr void hex20_dshape(int n, double xi, double eta, double zeta, double& dx, double& de, double& dz) {
     const double a = HXI[n], b = HETA[n], c = HZETA[n];
     if (n < 8) {
         dx = 0.125 * a * (1.0 + b * eta) * (1.0 + c * zeta) * (2.0 * a * xi + b * eta + c * zeta - 1.0);
@@ -89,6 +101,30 @@ __device__ __forceinline__ void hex20_dshape(int n, double xi, double eta, doubl
         dz = 0.25 * c * (1.0 + a * xi) * (1.0 - eta * eta);
     }
 }
+
+// Natural derivatives dN/d(xi, eta, zeta) of the 20 nodes at the 27 Gauss
+// points (g = 9 i + 3 j + k for xi_i, eta_j, zeta_k), evaluated at compile
+// time and laid out per gradient lane: lane (g, h), which takes nodes
+// 10h .. 10h+9, reads its 30 values from [(27 h + g) * 30].  They do not
+// depend on the element; 13 KB, cache-resident.
+struct alignas(16) DnTable {
+    double v[2 * 27 * 30];
+};
+constexpr DnTable make_dn_table() {
+    DnTable t{};
+    for (int h = 0; h < 2; ++h)
+        for (int g = 0; g < 27; ++g)
+            for (int k = 0; k < 10; ++k) {
+                double dx = 0, de = 0, dz = 0;
+                hex20_dshape(10 * h + k, HGP[g / 9], HGP[(g / 3) % 3], HGP[g % 3], dx, de, dz);
+                const int o = (27 * h + g) * 30 + 3 * k;
+                t.v[o] = dx;
+                t.v[o + 1] = de;
+                t.v[o + 2] = dz;
+            }
+    return t;
+}
+__constant__ DnTable HDN = make_dn_table();
 
 // Equal row nodes among the element's 20 node lanes: rank = lanes below
 // with the same node, cnt = lanes with it, lead = the lowest of them.
@@ -133,23 +169,72 @@ template <> struct OutRow<float> {
     static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
 };
 
+// Block (I, J) of the element matrix from the packed upper blocks of M
+// (block (J, I) transposed when I > J: M is symmetric):
+//   K_IJ = lam * M_IJ + mu * M_IJ^T + mu * tr(M_IJ) * I3.
+template <typename T>
+__device__ __forceinline__ void k_block(const T* Mb, int I, int Jn, T lam, T mu, T (&kb)[3][3]) {
+    const bool up = I <= Jn;
+    const int base = 9 * (up ? pair_index(I, Jn) : pair_index(Jn, I));
+    T mb[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) mb[r][c] = Mb[base + (up ? 3 * r + c : 3 * c + r)];
+    const T tr = mb[0][0] + mb[1][1] + mb[2][2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) kb[r][c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
+}
+
+// Rows of list values staged per pass of the insertion (LDS: rows * 720 B
+// in FP32, rows * 1440 B in FP64).
+template <typename T> struct Hex20Stage;
+template <> struct Hex20Stage<float> { static constexpr int rows = HEX20_STAGE_ROWS_F32; };
+template <> struct Hex20Stage<double> { static constexpr int rows = HEX20_STAGE_ROWS_F64; };
+
+// One 16-byte piece of a staged row strip from LDS to the list pool,
+// non-temporal: the merge reads it back once, and the pieces of a wave's
+// store instruction are contiguous (80^3, one box: FP64 insertion 7.3 ->
+// 5.0-5.7 ms and FP32 4.4 -> 2.9-3.0 ms against plain stores, two rows per
+// pass in FP64 and three in FP32; 5 and 10 rows measured slower).
+template <typename T>
+__device__ __forceinline__ void put_piece(T* dst, const T* src) {
+    typedef T v16 __attribute__((ext_vector_type(16 / sizeof(T))));
+    __builtin_nontemporal_store(*reinterpret_cast<const v16*>(src), reinterpret_cast<v16*>(dst));
+}
+
 // EMIT 0: append to the A-LLS lists; 1: FP64-atomic scatter into a cached
 // pattern; 2: plain "+=" scatter (one colour); 3: write each block to its
 // precomputed staging slot (value-only re-assembly, no atomics).
 template <typename T, int EMIT>
 __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter sc, int32_t* misc) {
-    constexpr int LDS_T = (2 * KS * XS > 60 * MS) ? 2 * KS * XS : 60 * MS;
-    __shared__ T lds[LDS_T];
+    __shared__ __align__(16) T lds[LDS_ELEMS];
     __shared__ double xyz[60];
+    __shared__ T wg[28];
     __shared__ int32_t nodes[20];
     __shared__ int32_t posv[20];
     __shared__ int64_t rbase[20];   // list index of each row group's first slot (-1: spans chunks, -2: none)
     __shared__ int32_t bad;
+    __shared__ __align__(16) T kst[EMIT == 0 ? Hex20Stage<T>::rows * 180 : 4];
 
     const int lane = threadIdx.x;
     const int64_t count = EMIT == 0 ? m.n_elems : sc.count;
-    // Grid-stride over the elements (a grid sized to the device, or one block per element).
-    for (int64_t idx = xcd_block(blockIdx.x, gridDim.x); idx < count; idx += gridDim.x) {
+    // Grid-stride over the elements, each XCD over its own slab.
+    const XcdRange xr = xcd_range(blockIdx.x, gridDim.x, count);
+    for (int64_t idx = xr.first; idx < xr.end; idx += xr.step) {
+        // The lane id, opaque to the compiler inside the loop: everything the
+        // loop derives from it (table loads, LDS indices) would otherwise be
+        // hoisted out and held in registers across the whole loop, which
+        // spills.
+        int ln = lane;
+        __asm__ volatile("" : "+v"(ln));
+        // Gradient phase: lane (g, h) takes Gauss point g < 27 and nodes
+        // 10h .. 10h+9 (lanes with g >= 27 compute point 26 and store nothing).
+        const int g = ln & 31, h = ln >> 5;
+        const int gq = g < 27 ? g : 26;
+        const double2* dn_src = reinterpret_cast<const double2*>(HDN.v + (27 * h + gq) * 30);
         const int64_t e = (EMIT != 0 && sc.elist) ? (int64_t)sc.elist[idx] : idx;
         if (lane == 0) bad = 0;
         __syncthreads();
@@ -160,6 +245,14 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         // register held across it cost 3.2 -> 3.6 ms there.
         constexpr bool early = EMIT == 0 && sizeof(T) == 8;
         const int32_t n = lane < 20 ? m.conn[e * m.npe + lane] : -1;
+        // This lane's natural derivatives (cache hits), in flight with the coordinates.
+        double dn[10][3];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) {
+            const double2 v = dn_src[k];
+            dn[(2 * k) / 3][(2 * k) % 3] = v.x;
+            dn[(2 * k + 1) / 3][(2 * k + 1) % 3] = v.y;
+        }
         const bool valid = lane < 20 && n >= 0 && n < m.n_nodes;
         // Lanes naming the same row node (an element that lists a node
         // twice) share one reservation: the lowest such lane reserves 20
@@ -193,64 +286,70 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
             continue;
         }
 
-        // Gauss-point gradients: lane g < 27 fills row g of X and W*X.
-        T* X = lds;
-        T* Y = lds + KS * XS;
-        if (lane < KS) {
-            if (lane < 27) {
-                const int gi = lane / 9, gj = (lane / 3) % 3, gk = lane % 3;
-                const double xi = HGP[gi], eta = HGP[gj], zeta = HGP[gk];
-                const double weight = HGW[gi] * HGW[gj] * HGW[gk];
-                double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                for (int n = 0; n < 20; ++n) {
-                    double dx, de, dz;
-                    hex20_dshape(n, xi, eta, zeta, dx, de, dz);
-                    const double px = xyz[3 * n], py = xyz[3 * n + 1], pz = xyz[3 * n + 2];
-                    J[0] += dx * px; J[1] += dx * py; J[2] += dx * pz;
-                    J[3] += de * px; J[4] += de * py; J[5] += de * pz;
-                    J[6] += dz * px; J[7] += dz * py; J[8] += dz * pz;
+        // Jacobian at Gauss point g: each half sums its 10 nodes, then the
+        // halves add (in either lane the same two terms, so the same bits).
+        T* Xt = lds;
+        {
+            double Jp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < 10; ++t) {
+                const int nd = 10 * h + t;
+                const double px = xyz[3 * nd], py = xyz[3 * nd + 1], pz = xyz[3 * nd + 2];
+                Jp[0] += dn[t][0] * px; Jp[1] += dn[t][0] * py; Jp[2] += dn[t][0] * pz;
+                Jp[3] += dn[t][1] * px; Jp[4] += dn[t][1] * py; Jp[5] += dn[t][1] * pz;
+                Jp[6] += dn[t][2] * px; Jp[7] += dn[t][2] * py; Jp[8] += dn[t][2] * pz;
+            }
+            double J[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) J[i] = Jp[i] + __shfl_xor(Jp[i], 32, 64);
+            double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                         J[2] * (J[3] * J[7] - J[4] * J[6]);
+            if (fabs(det) < 1e-15) det = 1e-15;  // Hex:156
+            const double r = 1.0 / det;
+            const double i0 = (J[4] * J[8] - J[5] * J[7]) * r, i1 = (J[2] * J[7] - J[1] * J[8]) * r;
+            const double i2 = (J[1] * J[5] - J[2] * J[4]) * r, i3 = (J[5] * J[6] - J[3] * J[8]) * r;
+            const double i4 = (J[0] * J[8] - J[2] * J[6]) * r, i5 = (J[2] * J[3] - J[0] * J[5]) * r;
+            const double i6 = (J[3] * J[7] - J[4] * J[6]) * r, i7 = (J[1] * J[6] - J[0] * J[7]) * r;
+            const double i8 = (J[0] * J[4] - J[1] * J[3]) * r;
+            if (g < 27) {
+                // Stored scaled by sqrt|w|, w = Gauss weight * det, with the
+                // sign of w kept apart: M = Z^T (sign(w) Z) takes one rounding
+                // per operand value, as a separate W X would, in one array.
+                const double w = HGW[g / 9] * HGW[(g / 3) % 3] * HGW[g % 3] * det;
+                const double sw = sqrt(fabs(w));
+#pragma unroll
+                for (int t = 0; t < 10; ++t) {
+                    const int c = 3 * (10 * h + t);
+                    const double gx = i0 * dn[t][0] + i1 * dn[t][1] + i2 * dn[t][2];
+                    const double gy = i3 * dn[t][0] + i4 * dn[t][1] + i5 * dn[t][2];
+                    const double gz = i6 * dn[t][0] + i7 * dn[t][1] + i8 * dn[t][2];
+                    Xt[(c + 0) * XT + g] = (T)(gx * sw);
+                    Xt[(c + 1) * XT + g] = (T)(gy * sw);
+                    Xt[(c + 2) * XT + g] = (T)(gz * sw);
                 }
-                double det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
-                             J[2] * (J[3] * J[7] - J[4] * J[6]);
-                if (fabs(det) < 1e-15) det = 1e-15;  // Hex:156
-                const double r = 1.0 / det;
-                const double i0 = (J[4] * J[8] - J[5] * J[7]) * r, i1 = (J[2] * J[7] - J[1] * J[8]) * r;
-                const double i2 = (J[1] * J[5] - J[2] * J[4]) * r, i3 = (J[5] * J[6] - J[3] * J[8]) * r;
-                const double i4 = (J[0] * J[8] - J[2] * J[6]) * r, i5 = (J[2] * J[3] - J[0] * J[5]) * r;
-                const double i6 = (J[3] * J[7] - J[4] * J[6]) * r, i7 = (J[1] * J[6] - J[0] * J[7]) * r;
-                const double i8 = (J[0] * J[4] - J[1] * J[3]) * r;
-                const double w = weight * det;
-                T* xr = X + lane * XS;
-                T* yr = Y + lane * XS;
-                for (int n = 0; n < 20; ++n) {
-                    double dx, de, dz;
-                    hex20_dshape(n, xi, eta, zeta, dx, de, dz);
-                    const double gx = i0 * dx + i1 * de + i2 * dz;
-                    const double gy = i3 * dx + i4 * de + i5 * dz;
-                    const double gz = i6 * dx + i7 * de + i8 * dz;
-                    xr[3 * n + 0] = (T)gx; xr[3 * n + 1] = (T)gy; xr[3 * n + 2] = (T)gz;
-                    yr[3 * n + 0] = (T)(w * gx); yr[3 * n + 1] = (T)(w * gy); yr[3 * n + 2] = (T)(w * gz);
-                }
-                for (int c = 60; c < XS; ++c) xr[c] = yr[c] = (T)0;
-            } else {
-                for (int c = 0; c < XS; ++c) X[lane * XS + c] = Y[lane * XS + c] = (T)0;
+                if (h == 0) wg[g] = w < 0.0 ? (T)-1 : (T)1;
             }
         }
         __syncthreads();
 
-        // M = X^T Y on the matrix cores: tiles (tr <= tc) of 16x16, A = X^T, B = Y.
+        // M = X^T (W X) = Z^T (S Z) on the matrix cores, Z = sqrt|W| X and S
+        // = sign(W): tiles (tr <= tc) of 16x16, A = Z^T and B = S Z from the
+        // same LDS word; k-row 27 (padding) is zero in both.
         typedef typename Acc<T>::type acc_t;
         acc_t acc[10];
     #pragma unroll
         for (int t = 0; t < 10; ++t) acc[t] = acc_t{0, 0, 0, 0};
-        const int li = lane & 15, lk = lane >> 4;
+        const int li = ln & 15, lk = ln >> 4;
     #pragma unroll
-        for (int s = 0; s < KS / 4; ++s) {
+        for (int s = 0; s < 7; ++s) {
+            const bool pad = s == 6 && lk == 3;
+            const T wk = pad ? (T)0 : wg[4 * s + lk];
             T av[4], bv[4];
     #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                av[t] = X[(4 * s + lk) * XS + 16 * t + li];
-                bv[t] = Y[(4 * s + lk) * XS + 16 * t + li];
+                const T x = Xt[(16 * t + li) * XT + 4 * s + lk];
+                av[t] = pad ? (T)0 : x;
+                bv[t] = av[t] * wk;
             }
             int t = 0;
     #pragma unroll
@@ -261,11 +360,23 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                     ++t;
                 }
         }
-        __syncthreads();   // X/Y are overwritten by M below
+        __syncthreads();   // Xt is overwritten by M below
 
-        // Scatter the tiles into M (row stride MS), mirroring the upper triangle.
-        T* M = lds;
+        // The tiles into the packed upper blocks of M.  Entry (a, b) of the
+        // product lies in node block (I, J) = (a / 3, b / 3) at (r, c) =
+        // (a % 3, b % 3); it is kept when I <= J, at RA(a) + CB(b) with
+        // RA = 9 * (I (39 - I) / 2) + 3 r and CB = 6 J + b.  A diagonal block
+        // (I == J) across two tiles has its lower entries in a tile that is
+        // not computed: the upper tile's entry also fills its mirror (c, r).
+        T* Mb = lds;
         {
+            int jb[4], cb[4];
+    #pragma unroll
+            for (int tc = 0; tc < 4; ++tc) {
+                const int b = 16 * tc + li;
+                jb[tc] = b / 3;
+                cb[tc] = 6 * jb[tc] + b;
+            }
             int t = 0;
     #pragma unroll
             for (int tr = 0; tr < 4; ++tr)
@@ -273,10 +384,12 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                 for (int tc = tr; tc < 4; ++tc) {
     #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const int r = 16 * tr + OutRow<T>::row(lk, q), c = 16 * tc + li;
-                        if (r < 60 && c < 60) {
-                            M[r * MS + c] = acc[t][q];
-                            if (tr != tc) M[c * MS + r] = acc[t][q];
+                        const int a = 16 * tr + OutRow<T>::row(lk, q), b = 16 * tc + li;
+                        const int I = a / 3, r = a - 3 * I, J = jb[tc];
+                        if (a < 60 && b < 60 && I <= J) {
+                            const int ra = 9 * (I * (39 - I) / 2) + 3 * r;
+                            Mb[ra + cb[tc]] = acc[t][q];
+                            if (tr != tc && I == J) Mb[ra + 9 * I - 2 * r + 3 * (b - 3 * J)] = acc[t][q];
                         }
                     }
                     ++t;
@@ -309,53 +422,61 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         }
         __syncthreads();
 
-        for (int pr = lane; pr < 400; pr += 64) {
-            const int I = pr / 20, Jn = pr % 20;
-            T mb[3][3];
+        if constexpr (EMIT == 0) {
+            // Rows of list values in passes of RS rows: the K blocks of the
+            // pass go to LDS in list order (a row's 20 entries are consecutive
+            // list slots), then consecutive lanes copy consecutive 16-byte
+            // pieces of each row strip, so every store instruction covers
+            // whole runs of memory.  A row whose strip is not 16-byte aligned
+            // in the pool (chunks that are not a multiple of 4 or 2 slots, or
+            // a strip across chunks) is written block by block instead.
+            constexpr int RS = Hex20Stage<T>::rows, VPP = 16 / (int)sizeof(T), PPR = 180 / VPP;
+            constexpr int64_t ALIGN = sizeof(T) == 4 ? 4 : 2;   // slots per 16 bytes of values
+            for (int r0 = 0; r0 < 20; r0 += RS) {
+                const int nr = 20 - r0 < RS ? 20 - r0 : RS;
+                for (int q = ln; q < nr * 20; q += 64) {
+                    const int ri = q / 20, I = r0 + ri, Jn = q - 20 * ri;
+                    T kb[3][3];
+                    k_block(Mb, I, Jn, lam, mu, kb);
+                    int64_t id = rbase[I];
+                    if (id == -2) continue;
+                    if (id >= 0 && id % ALIGN == 0) {
+                        T* st = kst + ri * 180 + Jn * 9;
     #pragma unroll
-            for (int r = 0; r < 3; ++r)
-    #pragma unroll
-                for (int c = 0; c < 3; ++c) mb[r][c] = M[(3 * I + r) * MS + 3 * Jn + c];
-            const T tr = mb[0][0] + mb[1][1] + mb[2][2];
-            T kb[3][3];
-    #pragma unroll
-            for (int r = 0; r < 3; ++r)
-    #pragma unroll
-                for (int c = 0; c < 3; ++c) kb[r][c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
-            if (EMIT == 0) {
-                int64_t id = rbase[I];
-                if (id == -2) continue;
-                if (id >= 0) {
-                    id += Jn;
-                } else {   // the group spans two chunks (C not a multiple of 20)
-                    const int32_t s = posv[I] + Jn;
-                    const int64_t ch = list_chunk(p, nodes[I], s);
-                    if (ch < 0) continue;
-                    id = ch * p.C + (s - chunk_of(p, s) * p.C);
-                }
-                T* dst = static_cast<T*>(p.vals) + id * 9;
-                // FP64 rows whose group starts at an even slot (every row with
-                // 160-slot chunks): lane pairs (2m, 2m+1) hold consecutive
-                // blocks of one row (20 is even), 144 bytes from a 16-byte
-                // boundary, written as nine 16-byte stores (80^3: 8.1 -> 7.55
-                // ms; at 80-slot chunks, where groups sat across chunks, the
-                // pairs measured 2.5% slower).  Both lanes of a pair take the
-                // same branch: the test depends on the row only.
-                bool paired = false;
-                if constexpr (sizeof(T) == 8) {
-                    if (rbase[I] >= 0 && (rbase[I] & 1) == 0) {
-                        put_block_pair(dst, kb, (pr & 1) == 0);
-                        paired = true;
+                        for (int v = 0; v < 9; ++v) st[v] = kb[v / 3][v % 3];
+                        p.keys[id + Jn] = make_int2(nodes[Jn], (int32_t)e);
+                        continue;
                     }
-                }
-                if (!paired) {
+                    if (id >= 0) {
+                        id += Jn;
+                    } else {   // the strip spans two chunks (C not a multiple of 20)
+                        const int32_t s = posv[I] + Jn;
+                        const int64_t ch = list_chunk(p, nodes[I], s);
+                        if (ch < 0) continue;
+                        id = ch * p.C + (s - chunk_of(p, s) * p.C);
+                    }
+                    T* dst = static_cast<T*>(p.vals) + id * 9;
     #pragma unroll
                     for (int r = 0; r < 3; ++r)
     #pragma unroll
-                        for (int c = 0; c < 3; ++c) put_block_value(dst + r * 3 + c, kb[r][c]);
+                        for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
+                    p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
                 }
-                p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
-            } else if (EMIT == 3) {
+                __syncthreads();
+                for (int pc = ln; pc < nr * PPR; pc += 64) {
+                    const int ri = pc / PPR, off = pc - ri * PPR;
+                    const int64_t rb = rbase[r0 + ri];
+                    if (rb >= 0 && rb % ALIGN == 0)
+                        put_piece(static_cast<T*>(p.vals) + rb * 9 + off * VPP, kst + ri * 180 + off * VPP);
+                }
+                __syncthreads();
+            }
+        } else {
+        for (int pr = ln; pr < 400; pr += 64) {
+            const int I = pr / 20, Jn = pr % 20;
+            T kb[3][3];
+            k_block(Mb, I, Jn, lam, mu, kb);
+            if (EMIT == 3) {
                 T* dst = static_cast<T*>(sc.stage) + (e * 400 + pr) * 9;
                 if constexpr (sizeof(T) == 8) {
                     put_block_pair(dst, kb, (pr & 1) == 0);   // pr and lane have the same parity
@@ -380,6 +501,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                     }
             }
         }
+        }
         __syncthreads();   // LDS is refilled by the next element
     }
 }
@@ -388,11 +510,15 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
 
 namespace alls {
 
+// Grid of k_hex20: one block per element, or max_blocks > 0 (persistent).
+static int64_t hex20_grid(int64_t count, int64_t max_blocks) {
+    return max_blocks > 0 && count > max_blocks ? max_blocks : count;
+}
+
 void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, int64_t max_blocks, hipStream_t s) {
     if (m.n_elems <= 0) return;
     const Scatter none{};
-    int64_t g = m.n_elems;
-    if (max_blocks > 0 && g > max_blocks) g = max_blocks;
+    const int64_t g = hex20_grid(m.n_elems, max_blocks);
     if (f32)
         hipLaunchKernelGGL((k_hex20<float, 0>), dim3((unsigned)g), dim3(64), 0, s, m, p, none, p.misc);
     else
@@ -402,7 +528,7 @@ void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, int64_t m
 void launch_scatter_hex20(const DevMesh& m, const Scatter& sc, int mode, int f32, int32_t* misc, hipStream_t s) {
     if (sc.count <= 0) return;
     const ListPool none{};
-    const dim3 g((unsigned)sc.count);
+    const dim3 g((unsigned)hex20_grid(sc.count, 0));
 #define ALLS_HX(T, E) hipLaunchKernelGGL((k_hex20<T, E>), g, dim3(64), 0, s, m, none, sc, misc)
     if (f32) {
         if (mode == 1) ALLS_HX(float, 1);
