@@ -22,38 +22,6 @@ namespace {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// A block value into the lists or the element-order stage, read once by a
-// later kernel: non-temporal in FP32 (insertion 4.59 -> 4.35 ms, value-only
-// 7.35 -> 7.08 ms at 80^3), plain in FP64, where the 8-byte non-temporal
-// stores did not combine and the insertion took 22.5 ms instead of 7.9.
-template <typename T>
-__device__ __forceinline__ void put_block_value(T* dst, T x) {
-    if constexpr (sizeof(T) == 4) __builtin_nontemporal_store(x, dst);
-    else *dst = x;
-}
-
-// FP64 blocks of lanes 2m and 2m+1, which hold consecutive entries of the
-// element-order stage (9 doubles each; the pair starts 16-byte aligned): the pair's
-// 144 bytes go out as nine 16-byte stores, five from the even lane (its
-// block and the odd lane's first value) and four from the odd lane, instead
-// of eighteen 8-byte stores.  `dst` is the lane's own entry; both lanes of a
-// pair must be active.
-__device__ __forceinline__ void put_block_pair(double* dst, const double (&kb)[3][3], bool even) {
-    const double first_of_odd = __shfl_xor(kb[0][0], 1, 64);
-    double2* d = reinterpret_cast<double2*>(even ? dst : dst + 1);
-    if (even) {
-        d[0] = make_double2(kb[0][0], kb[0][1]);
-        d[1] = make_double2(kb[0][2], kb[1][0]);
-        d[2] = make_double2(kb[1][1], kb[1][2]);
-        d[3] = make_double2(kb[2][0], kb[2][1]);
-        d[4] = make_double2(kb[2][2], first_of_odd);
-    } else {
-        d[0] = make_double2(kb[0][1], kb[0][2]);
-        d[1] = make_double2(kb[1][0], kb[1][1]);
-        d[2] = make_double2(kb[1][2], kb[2][0]);
-        d[3] = make_double2(kb[2][1], kb[2][2]);
-    }
-}
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // Gauss rule exactly as the reference writes it (Hex:45-46).
@@ -217,7 +185,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
     __shared__ int32_t posv[20];
     __shared__ int64_t rbase[20];   // list index of each row group's first slot (-1: spans chunks, -2: none)
     __shared__ int32_t bad;
-    __shared__ __align__(16) T kst[EMIT == 0 ? Hex20Stage<T>::rows * 180 : 4];
+    __shared__ __align__(16) T kst[EMIT == 0 || EMIT == 3 ? Hex20Stage<T>::rows * 180 : 4];
 
     const int lane = threadIdx.x;
     const int64_t count = EMIT == 0 ? m.n_elems : sc.count;
@@ -419,17 +387,22 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                 }
                 rbase[lane] = b;
             }
+        } else if (EMIT == 3) {
+            // Element-order staging: row I's 20 blocks at entries e*400 + 20 I.
+            if (lane < 20) rbase[lane] = e * 400 + 20 * lane;
         }
         __syncthreads();
 
-        if constexpr (EMIT == 0) {
+        if constexpr (EMIT == 0 || EMIT == 3) {
             // Rows of list values in passes of RS rows: the K blocks of the
             // pass go to LDS in list order (a row's 20 entries are consecutive
             // list slots), then consecutive lanes copy consecutive 16-byte
             // pieces of each row strip, so every store instruction covers
             // whole runs of memory.  A row whose strip is not 16-byte aligned
             // in the pool (chunks that are not a multiple of 4 or 2 slots, or
-            // a strip across chunks) is written block by block instead.
+            // a strip across chunks) is written block by block instead.  The
+            // value-only path (EMIT 3) writes the same strips, in element
+            // order, to the staging array (always aligned, no keys).
             constexpr int RS = Hex20Stage<T>::rows, VPP = 16 / (int)sizeof(T), PPR = 180 / VPP;
             constexpr int64_t ALIGN = sizeof(T) == 4 ? 4 : 2;   // slots per 16 bytes of values
             for (int r0 = 0; r0 < 20; r0 += RS) {
@@ -440,11 +413,11 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                     k_block(Mb, I, Jn, lam, mu, kb);
                     int64_t id = rbase[I];
                     if (id == -2) continue;
-                    if (id >= 0 && id % ALIGN == 0) {
+                    if (EMIT == 3 || (id >= 0 && id % ALIGN == 0)) {
                         T* st = kst + ri * 180 + Jn * 9;
     #pragma unroll
                         for (int v = 0; v < 9; ++v) st[v] = kb[v / 3][v % 3];
-                        p.keys[id + Jn] = make_int2(nodes[Jn], (int32_t)e);
+                        if (EMIT == 0) p.keys[id + Jn] = make_int2(nodes[Jn], (int32_t)e);
                         continue;
                     }
                     if (id >= 0) {
@@ -466,8 +439,8 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                 for (int pc = ln; pc < nr * PPR; pc += 64) {
                     const int ri = pc / PPR, off = pc - ri * PPR;
                     const int64_t rb = rbase[r0 + ri];
-                    if (rb >= 0 && rb % ALIGN == 0)
-                        put_piece(static_cast<T*>(p.vals) + rb * 9 + off * VPP, kst + ri * 180 + off * VPP);
+                    T* dst = static_cast<T*>(EMIT == 0 ? p.vals : sc.stage);
+                    if (rb >= 0 && rb % ALIGN == 0) put_piece(dst + rb * 9 + off * VPP, kst + ri * 180 + off * VPP);
                 }
                 __syncthreads();
             }
@@ -476,17 +449,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
             const int I = pr / 20, Jn = pr % 20;
             T kb[3][3];
             k_block(Mb, I, Jn, lam, mu, kb);
-            if (EMIT == 3) {
-                T* dst = static_cast<T*>(sc.stage) + (e * 400 + pr) * 9;
-                if constexpr (sizeof(T) == 8) {
-                    put_block_pair(dst, kb, (pr & 1) == 0);   // pr and lane have the same parity
-                } else {
-    #pragma unroll
-                    for (int r = 0; r < 3; ++r)
-    #pragma unroll
-                        for (int c = 0; c < 3; ++c) put_block_value(dst + r * 3 + c, kb[r][c]);
-                }
-            } else {
+            {
                 const int64_t a = nodes[I];
                 const int64_t rs = rp_at(sc.row_ptr, sc.i64, 3 * a);
                 const int64_t rowlen = rp_at(sc.row_ptr, sc.i64, 3 * a + 1) - rs;
