@@ -33,17 +33,20 @@ constexpr double HETA[20] = {-1, -1, 1, 1, -1, -1, 1, 1, -1, 0, 1, 0, -1, 0, 1, 
 constexpr double HZETA[20] = {-1, -1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 1, 1, 1, 1, 0, 0, 0, 0};
 
 // LDS of one element (one wave), two phases over the same words:
-//   gradients, transposed and scaled: Xt[c * XT + g] = sqrt|w_g| *
-//   dN_{c/3}/dx_{c%3} at Gauss point g (c < 60; rows 60..63 are read by the
-//   padded MFMA tiles and only reach discarded outputs; the k-padding row
-//   g = 27 is masked at the load);
+//   gradients, transposed and scaled: Xt[P * XT + g] = sqrt|w_g| *
+//   dN_n/dx_p at Gauss point g, at position P = 16 (n / 5) + 3 (n % 5) + p,
+//   so that each 16-wide MFMA tile holds five whole nodes (positions 15, 31,
+//   47 and 63 are never written: they only reach output rows and columns
+//   that are discarded; the k-padding row g = 27 is masked at the load);
 //   M as its 210 upper 3x3 node blocks, pair (I <= J) at 9 * pair_index(I, J),
-//   row-major inside the block.
+//   row-major inside the block, then 64 words where lanes park the product
+//   entries that belong to no stored block (branch-free stores).
 constexpr int XT = 29;                  // Gauss points 27, padded to 28 k-rows, +1 against bank conflicts
 constexpr int NPAIR = 210;              // node pairs I <= J
+constexpr int MB_DUMP = NPAIR * 9;      // first of the 64 parking words
 constexpr int HEX20_STAGE_ROWS_F32 = 3;   // node rows of list values staged per pass
 constexpr int HEX20_STAGE_ROWS_F64 = 2;
-constexpr int LDS_ELEMS = NPAIR * 9 > 64 * XT ? NPAIR * 9 : 64 * XT;
+constexpr int LDS_ELEMS = MB_DUMP + 64 > 64 * XT ? MB_DUMP + 64 : 64 * XT;
 
 // Index of node pair (I, J), I <= J, in the packed upper triangle.
 __device__ __forceinline__ int pair_index(int I, int J) { return I * (39 - I) / 2 + J; }
@@ -287,7 +290,7 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                 const double sw = sqrt(fabs(w));
 #pragma unroll
                 for (int t = 0; t < 10; ++t) {
-                    const int c = 3 * (10 * h + t);
+                    const int c = 32 * h + 16 * (t / 5) + 3 * (t % 5);   // node 10h + t
                     const double gx = i0 * dn[t][0] + i1 * dn[t][1] + i2 * dn[t][2];
                     const double gy = i3 * dn[t][0] + i4 * dn[t][1] + i5 * dn[t][2];
                     const double gz = i6 * dn[t][0] + i7 * dn[t][1] + i8 * dn[t][2];
@@ -330,21 +333,13 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         }
         __syncthreads();   // Xt is overwritten by M below
 
-        // The tiles into the packed upper blocks of M.  Entry (a, b) of the
-        // product lies in node block (I, J) = (a / 3, b / 3) at (r, c) =
-        // (a % 3, b % 3); it is kept when I <= J, at RA(a) + CB(b) with
-        // RA = 9 * (I (39 - I) / 2) + 3 r and CB = 6 J + b.  A diagonal block
-        // (I == J) across two tiles has its lower entries in a tile that is
-        // not computed: the upper tile's entry also fills its mirror (c, r).
+        // The tiles into the packed upper blocks of M.  Entry (r16, c16) of
+        // tile (tr, tc) is entry (rr, cc) = (r16 % 3, c16 % 3) of node block
+        // (I, J) = (5 tr + r16 / 3, 5 tc + c16 / 3) (r16, c16 < 15), kept when
+        // I <= J, at 9 pair_index(I, J) + 3 rr + cc; no block crosses a tile.
         T* Mb = lds;
         {
-            int jb[4], cb[4];
-    #pragma unroll
-            for (int tc = 0; tc < 4; ++tc) {
-                const int b = 16 * tc + li;
-                jb[tc] = b / 3;
-                cb[tc] = 6 * jb[tc] + b;
-            }
+            const int jl = (li * 11) >> 5, cc = li - 3 * jl;   // li / 3, li % 3 for li < 16
             int t = 0;
     #pragma unroll
             for (int tr = 0; tr < 4; ++tr)
@@ -352,13 +347,12 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
                 for (int tc = tr; tc < 4; ++tc) {
     #pragma unroll
                     for (int q = 0; q < 4; ++q) {
-                        const int a = 16 * tr + OutRow<T>::row(lk, q), b = 16 * tc + li;
-                        const int I = a / 3, r = a - 3 * I, J = jb[tc];
-                        if (a < 60 && b < 60 && I <= J) {
-                            const int ra = 9 * (I * (39 - I) / 2) + 3 * r;
-                            Mb[ra + cb[tc]] = acc[t][q];
-                            if (tr != tc && I == J) Mb[ra + 9 * I - 2 * r + 3 * (b - 3 * J)] = acc[t][q];
-                        }
+                        const int r16 = OutRow<T>::row(lk, q);
+                        const int il = (r16 * 11) >> 5, rr = r16 - 3 * il;
+                        const int I = 5 * tr + il, J = 5 * tc + jl;
+                        const bool keep = li < 15 && r16 < 15 && (tr < tc || il <= jl);
+                        const int at = 9 * pair_index(I, J) + 3 * rr + cc;
+                        Mb[keep ? at : MB_DUMP + lane] = acc[t][q];
                     }
                     ++t;
                 }
