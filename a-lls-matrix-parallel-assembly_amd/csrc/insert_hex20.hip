@@ -179,8 +179,13 @@ __device__ __forceinline__ void put_piece(T* dst, const T* src) {
 // EMIT 0: append to the A-LLS lists; 1: FP64-atomic scatter into a cached
 // pattern; 2: plain "+=" scatter (one colour); 3: write each block to its
 // precomputed staging slot (value-only re-assembly, no atomics).
+// FP32 asks for 4 waves per SIMD (128 VGPRs, no spills; LDS allows 14 per
+// CU), FP64 for 2 (150 VGPRs; 19 KB of LDS allows 8 per CU): FP32 insertion
+// 2.95 -> 2.77 ms at 80^3 together with its early reservations (three
+// interleaved rounds on one box).
 template <typename T, int EMIT>
-__global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter sc, int32_t* misc) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_hex20(
+    DevMesh m, ListPool p, Scatter sc, int32_t* misc) {
     __shared__ __align__(16) T lds[LDS_ELEMS];
     __shared__ double xyz[60];
     __shared__ T wg[28];
@@ -209,12 +214,10 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         const int64_t e = (EMIT != 0 && sc.elist) ? (int64_t)sc.elist[idx] : idx;
         if (lane == 0) bad = 0;
         __syncthreads();
-        // FP64: the reservation atomics go out with the coordinate loads, so
-        // their round trip overlaps the integration (80^3: 8.1 -> 7.8 ms; an
-        // element with a bad node makes the whole call fail, so its reserved
-        // slots are never read).  FP32 reserves after the contraction: the
-        // register held across it cost 3.2 -> 3.6 ms there.
-        constexpr bool early = EMIT == 0 && sizeof(T) == 8;
+        // The reservation atomics go out with the coordinate loads, so their
+        // round trip overlaps the integration (an element with a bad node
+        // makes the whole call fail, so its reserved slots are never read).
+        constexpr bool early = EMIT == 0;
         const int32_t n = lane < 20 ? m.conn[e * m.npe + lane] : -1;
         // This lane's natural derivatives (cache hits), in flight with the coordinates.
         double dn[10][3];
@@ -362,9 +365,8 @@ __global__ __launch_bounds__(64, 2) void k_hex20(DevMesh m, ListPool p, Scatter 
         const T lam = (T)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
         const T mu = (T)(E / (2.0 * (1.0 + nu)));
         if (EMIT == 0) {
-            // Reserve 20 slots in each of the element's 20 node rows.
+            // The 20 slots reserved in each of the element's 20 node rows.
             if (lane < 20 && rk == 0) {
-                if (!early) pos = list_reserve_issue(p, nodes[lane], 20 * cnt);
                 list_note_long(p, pos, 20 * cnt);
                 list_own_chunks(p, nodes[lane], pos, 20 * cnt);
             }
