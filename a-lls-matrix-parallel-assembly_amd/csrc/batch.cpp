@@ -48,7 +48,7 @@ static alls_status tune(alls_ctx* ctx, const DevMesh& dm, const MeshInfo& mi, co
     std::vector<int> blocks, grids, chunks;
     if (mi.kind == 3) {
         blocks = {64};
-        grids = {0, 4, 8};
+        grids = {0, 16, 32};
         chunks = {40, 80, 160};
     } else {
         blocks = {64, 128, 256};
@@ -280,7 +280,10 @@ alls_status batch_assemble(Workspace& ws, hipStream_t s, int32_t n, const alls_m
     dm.dim = dim;
     dm.npe = npe;
     dm.uniform_type = uniform;
-    if (o.grid_per_cu <= 0 && !o.autotune) o.grid_per_cu = dim == 3 ? 4 : 8;   // persistent by default
+    // Persistent by default: 32 blocks per CU for Hex20 (a 110k-element batch
+    // of 8 meshes: FP64 3.60 -> 2.64 ms, FP32 2.73 -> 2.05 ms against 4 per
+    // CU; 64^3: 8.56 -> 6.25 / 6.36 -> 4.74 ms), 8 for 2D.
+    if (o.grid_per_cu <= 0 && !o.autotune) o.grid_per_cu = dim == 3 ? 32 : 8;
     alls_status st = ALLS_OK;
     if (tune_ctx) st = apply_tuning(tune_ctx, dm, mi, &o);
     if (st != ALLS_OK) return st;

@@ -230,7 +230,7 @@ alls_status alls_assemble_colored(alls_ctx* ctx, const alls_pattern* pat, const 
  * packed into one node/element space (mesh k's node ids shifted by the nodes
  * of meshes 0..k-1), so each stage is ONE launch over all of them whatever
  * their sizes; the insertion kernel runs persistent (opts->grid_per_cu,
- * default 8 blocks per CU for 2D and 4 for Hex20); and the whole pipeline
+ * default 8 blocks per CU for 2D and 32 for Hex20); and the whole pipeline
  * runs as ONE hipGraph (into an output sized by the list entries bound, so
  * no host read-back splits it), captured on the first call and replayed
  * while the batch keeps its shape (sizes and material layout; new
