@@ -171,9 +171,16 @@ template <> struct Hex20Stage<double> { static constexpr int rows = HEX20_STAGE_
 // 5.0-5.7 ms and FP32 4.4 -> 2.9-3.0 ms against plain stores, two rows per
 // pass in FP64 and three in FP32; 5 and 10 rows measured slower).
 template <typename T>
-__device__ __forceinline__ void put_piece(T* dst, const T* src) {
-    typedef T v16 __attribute__((ext_vector_type(16 / sizeof(T))));
-    __builtin_nontemporal_store(*reinterpret_cast<const v16*>(src), reinterpret_cast<v16*>(dst));
+struct Piece {
+    typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+};
+template <typename T>
+__device__ __forceinline__ typename Piece<T>::type get_piece(const T* src) {
+    return *reinterpret_cast<const typename Piece<T>::type*>(src);
+}
+template <typename T>
+__device__ __forceinline__ void put_piece(T* dst, typename Piece<T>::type v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<typename Piece<T>::type*>(dst));
 }
 
 // EMIT 0: append to the A-LLS lists; 1: FP64-atomic scatter into a cached
@@ -401,9 +408,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
             // order, to the staging array (always aligned, no keys).
             constexpr int RS = Hex20Stage<T>::rows, VPP = 16 / (int)sizeof(T), PPR = 180 / VPP;
             constexpr int64_t ALIGN = sizeof(T) == 4 ? 4 : 2;   // slots per 16 bytes of values
+            // Fixed trip counts per pass (KIT K-formation and CIT copy-out
+            // steps of 64 lanes), so that the compiler issues a pass's LDS
+            // reads together instead of one round trip per 16-byte piece.
+            constexpr int KIT = (RS * 20 + 63) / 64, CIT = (RS * PPR + 63) / 64;
+            T* dst = static_cast<T*>(EMIT == 0 ? p.vals : sc.stage);
             for (int r0 = 0; r0 < 20; r0 += RS) {
                 const int nr = 20 - r0 < RS ? 20 - r0 : RS;
-                for (int q = ln; q < nr * 20; q += 64) {
+    #pragma unroll
+                for (int it = 0; it < KIT; ++it) {
+                    const int q = ln + 64 * it;
+                    if (q >= nr * 20) break;
                     const int ri = q / 20, I = r0 + ri, Jn = q - 20 * ri;
                     T kb[3][3];
                     k_block(Mb, I, Jn, lam, mu, kb);
@@ -424,19 +439,34 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                         if (ch < 0) continue;
                         id = ch * p.C + (s - chunk_of(p, s) * p.C);
                     }
-                    T* dst = static_cast<T*>(p.vals) + id * 9;
+                    T* d = static_cast<T*>(p.vals) + id * 9;
     #pragma unroll
                     for (int r = 0; r < 3; ++r)
     #pragma unroll
-                        for (int c = 0; c < 3; ++c) dst[r * 3 + c] = kb[r][c];
+                        for (int c = 0; c < 3; ++c) d[r * 3 + c] = kb[r][c];
                     p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
                 }
                 __syncthreads();
-                for (int pc = ln; pc < nr * PPR; pc += 64) {
+                // The pass's row bases, read once (uniform per row), and all
+                // of its pieces read from LDS (index clamped into the stage)
+                // before any is stored.
+                int64_t rbp[RS];
+    #pragma unroll
+                for (int r = 0; r < RS; ++r) rbp[r] = rbase[r0 + r < 20 ? r0 + r : 19];
+                typename Piece<T>::type pv[CIT];
+    #pragma unroll
+                for (int it = 0; it < CIT; ++it) {
+                    const int pc = min(ln + 64 * it, RS * PPR - 1);
+                    pv[it] = get_piece(kst + pc * VPP);   // piece pc of the stage (rows are 180 values, PPR pieces)
+                }
+    #pragma unroll
+                for (int it = 0; it < CIT; ++it) {
+                    const int pc = ln + 64 * it;
                     const int ri = pc / PPR, off = pc - ri * PPR;
-                    const int64_t rb = rbase[r0 + ri];
-                    T* dst = static_cast<T*>(EMIT == 0 ? p.vals : sc.stage);
-                    if (rb >= 0 && rb % ALIGN == 0) put_piece(dst + rb * 9 + off * VPP, kst + ri * 180 + off * VPP);
+                    int64_t rb = rbp[0];
+    #pragma unroll
+                    for (int r = 1; r < RS; ++r) rb = ri == r ? rbp[r] : rb;
+                    if (pc < nr * PPR && rb >= 0 && rb % ALIGN == 0) put_piece(dst + rb * 9 + off * VPP, pv[it]);
                 }
                 __syncthreads();
             }
