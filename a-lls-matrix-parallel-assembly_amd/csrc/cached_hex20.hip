@@ -17,50 +17,100 @@
 
 namespace {
 
-// Wave per node row; acc (LDS) holds the row's u column blocks.
-template <typename T, typename RP>
+// Wave per node row; acc (LDS) holds the row's u column blocks.  A row of
+// at most 3 * KS incident elements (every row of a conforming hexahedral
+// mesh has at most 8) is taken in three dependent rounds of loads: the
+// element ids (lane l: the row's l-th element), then their node lists
+// (register slot k, lane (sub, J): node J of element 3k + sub), then each
+// block's rank and values; the blocks are then added one element after the
+// other.  Longer rows take three elements per round.
+template <typename T, typename RP, int KS>
 __global__ __launch_bounds__(64) void k_pair_gather(int64_t n_nodes, const int32_t* conn, const RP* rp,
                                                     const int32_t* rank_map, const int64_t* inc_off,
                                                     const int32_t* inc_list, const T* stage, double* val) {
     extern __shared__ double acc[];   // [u][9]
     const int lane = threadIdx.x;
-    const int sub = lane / 20, J = lane - 20 * sub;   // element of the step (0..2; 3 = idle), column node
+    const int sub = lane / 20, J = lane - 20 * sub;   // element of the slot (0..2; 3 = idle), column node
     for (int64_t a = xcd_block(blockIdx.x, gridDim.x); a < n_nodes; a += gridDim.x) {
         const int64_t r0 = (int64_t)rp[3 * a], r1 = (int64_t)rp[3 * a + 1], r2 = (int64_t)rp[3 * a + 2];
         const int64_t u = (r1 - r0) / 3;
+        const int64_t k0 = inc_off[a], k1 = inc_off[a + 1];
+        const int64_t ne = k1 - k0;
         for (int64_t i = lane; i < u * 9; i += 64) acc[i] = 0.0;
         __syncthreads();
-        const int64_t k0 = inc_off[a], k1 = inc_off[a + 1];
-        for (int64_t k = k0; k < k1; k += 3) {
-            const int64_t kk = k + sub;
-            const bool active = sub < 3 && kk < k1;
-            int64_t e = 0;
-            int32_t cj = -1;
-            if (active) {
-                e = inc_list[kk];
-                cj = conn[e * 20 + J];
+        if (ne <= 3 * KS) {
+            const int32_t el = lane < ne ? inc_list[k0 + lane] : 0;
+            int64_t e[KS];
+            int32_t cj[KS];
+#pragma unroll
+            for (int k = 0; k < KS; ++k) {
+                const int idx = 3 * k + sub;
+                e[k] = (int64_t)__shfl(el, idx < 64 ? idx : 0, 64);
+                cj[k] = (sub < 3 && idx < ne) ? conn[e[k] * 20 + J] : -1;
             }
-            // Local index of node a in each of the three elements.
-            const uint64_t hit = __ballot(active && cj == a);
-            const uint32_t mine = (uint32_t)(hit >> (20 * (sub < 3 ? sub : 0))) & 0xfffffu;
-            const int I = mine ? __ffs(mine) - 1 : -1;
-            int32_t r = -1;
-            double v[9];
-            if (active && I >= 0) {
-                const int64_t b = e * 400 + I * 20 + J;
-                r = rank_map[b];
-                const T* src = stage + b * 9;
+            int32_t r[KS];
+            double v[KS][9];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) v[q] = (double)src[q];
-            }
-            // In element order: the three elements add one after the other.
+            for (int k = 0; k < KS; ++k) {
+                // Local index of node a in element 3k + sub.
+                const uint64_t hit = __ballot(cj[k] == a);
+                const uint32_t mine = (uint32_t)(hit >> (20 * (sub < 3 ? sub : 0))) & 0xfffffu;
+                const int I = (sub < 3 && mine) ? __ffs(mine) - 1 : -1;
+                r[k] = -1;
 #pragma unroll
-            for (int s = 0; s < 3; ++s) {
-                if (sub == s && r >= 0) {
+                for (int q = 0; q < 9; ++q) v[k][q] = 0.0;
+                if (I >= 0) {
+                    const int64_t b = e[k] * 400 + I * 20 + J;
+                    r[k] = rank_map[b];
+                    const T* src = stage + b * 9;
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) acc[r * 9 + q] += v[q];
+                    for (int q = 0; q < 9; ++q) v[k][q] = (double)src[q];
                 }
-                __syncthreads();
+            }
+            // In element order: one element's 20 blocks (distinct columns) per step.
+#pragma unroll
+            for (int k = 0; k < KS; ++k)
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    if (3 * k + s >= ne) break;
+                    if (sub == s && r[k] >= 0) {
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) acc[r[k] * 9 + q] += v[k][q];
+                    }
+                    __syncthreads();
+                }
+        } else {
+            for (int64_t k = k0; k < k1; k += 3) {
+                const int64_t kk = k + sub;
+                const bool active = sub < 3 && kk < k1;
+                int64_t e = 0;
+                int32_t cj = -1;
+                if (active) {
+                    e = inc_list[kk];
+                    cj = conn[e * 20 + J];
+                }
+                // Local index of node a in each of the three elements.
+                const uint64_t hit = __ballot(active && cj == a);
+                const uint32_t mine = (uint32_t)(hit >> (20 * (sub < 3 ? sub : 0))) & 0xfffffu;
+                const int I = mine ? __ffs(mine) - 1 : -1;
+                int32_t r = -1;
+                double v[9];
+                if (active && I >= 0) {
+                    const int64_t b = e * 400 + I * 20 + J;
+                    r = rank_map[b];
+                    const T* src = stage + b * 9;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) v[q] = (double)src[q];
+                }
+                // In element order: the three elements add one after the other.
+#pragma unroll
+                for (int s = 0; s < 3; ++s) {
+                    if (sub == s && r >= 0) {
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) acc[r * 9 + q] += v[q];
+                    }
+                    __syncthreads();
+                }
             }
         }
         const int64_t rows[3] = {r0, r1, r2};
@@ -112,7 +162,7 @@ void launch_pair_gather(int64_t n_nodes, const int32_t* conn, const void* rp, in
     if (blocks > 262144) blocks = 262144;
     const size_t lds = (size_t)std::max(umax, 1) * 9 * sizeof(double);
 #define ALLS_PG(T, RP)                                                                                      \
-    hipLaunchKernelGGL((k_pair_gather<T, RP>), dim3((unsigned)blocks), dim3(64), lds, s, n_nodes, conn,     \
+    hipLaunchKernelGGL((k_pair_gather<T, RP, 3>), dim3((unsigned)blocks), dim3(64), lds, s, n_nodes, conn,     \
                        static_cast<const RP*>(rp), rank_map, inc_off, inc_list, static_cast<const T*>(stage), val)
     if (f32) {
         if (i64) ALLS_PG(float, int64_t);
