@@ -81,6 +81,37 @@ def test_value_only_output_reuse_and_pattern_switch(ctx):
     compare_csr(out.download(), oracle.assemble(a))
 
 
+@pytest.mark.parametrize("prec", [alls.PREC_F64, alls.PREC_F32_MFMA], ids=["f64", "f32"])
+def test_hex20_value_only_rows_with_many_elements(ctx, prec):
+    """Every element of a 2x2x2 brick listed twice: the centre vertex row
+    has 16 incident elements, more than the value-only gather takes in one
+    round of loads (9), so it goes through the three-at-a-time loop; the
+    edge and face rows (8 or fewer) through the one-round path.  Both equal
+    the plain assembly bitwise, and the oracle."""
+    m0 = alls.generate(alls.GEN_HEX20, 2, 2, 2, jitter=0.1, seed=11)
+    conn = m0.conn.reshape(-1, 20)
+    conn2 = np.concatenate([conn, conn[::-1]])
+    E = np.exp(0.5 * np.random.default_rng(5).standard_normal(conn2.shape[0]))
+    m = alls.MeshArrays(m0.coords, conn2.reshape(-1), 3, 20, m0.uniform_type, None, 1.0, E, m0.nu,
+                        m0.thickness)
+    inc = np.bincount(conn2.reshape(-1), minlength=m.n_nodes)
+    assert inc.max() == 16 and ((inc > 0) & (inc <= 9)).any()
+    plain = ctx.assemble(m, precision=prec)
+    got = plain.download()
+    plain.free()
+    ref = oracle.assemble(m)
+    if prec == alls.PREC_F64:
+        compare_csr(got, ref)
+    else:
+        assert np.linalg.norm(got[2] - ref[2]) / np.linalg.norm(ref[2]) < 1e-6
+    pat = alls.Pattern(ctx, m, precision=prec)
+    out = pat.values(E)
+    for a, b in zip(out.download(), got):
+        assert np.array_equal(a, b)
+    out.free()
+    pat.free()
+
+
 def test_hex20_f32_value_only(ctx):
     m = alls.generate(alls.GEN_HEX20, 4, 3, 3, jitter=0.15, seed=7)
     pat = alls.Pattern(ctx, m, precision=alls.PREC_F32_MFMA)
