@@ -313,6 +313,38 @@ __device__ __forceinline__ int32_t nd_at(const Elem2D& el, int i) {
     return i == 0 ? el.nd[0] : i == 1 ? el.nd[1] : i == 2 ? el.nd[2] : el.nd[3];
 }
 
+// Integrates element e (loaded) for row node a and adds its blocks into the
+// row's register accumulators (block slots from the cached map).
+__device__ __forceinline__ void gather_add_element(Elem2D& el, int64_t e, int64_t a, const Scatter& sc,
+                                                   double (&acc)[GATHER_SLOTS][4]) {
+    el.material();
+    // Only row node i's moments are formed (i is a run-time index); the
+    // accumulators are indexed by constants only and stay in registers.
+#pragma unroll 1
+    for (int i = 0; i < el.nn; ++i) {
+        if (nd_at(el, i) != a) continue;
+        RowMoments rm;
+        el.row_moments(i, rm);
+        const int4 mp = *reinterpret_cast<const int4*>(sc.map + e * 16 + i * 4);
+        const int32_t sl[4] = {mp.x, mp.y, mp.z, mp.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= el.nn) continue;
+            double kb[4];
+            el.row_block(rm, j, kb);
+#pragma unroll
+            for (int q = 0; q < GATHER_SLOTS; ++q) {
+                if (sl[j] == q) {
+                    acc[q][0] += kb[0];
+                    acc[q][1] += kb[1];
+                    acc[q][2] += kb[2];
+                    acc[q][3] += kb[3];
+                }
+            }
+        }
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const int64_t* inc_off,
                                                    const int32_t* inc_list, int32_t* misc) {
@@ -326,40 +358,25 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
         double acc[GATHER_SLOTS][4];
 #pragma unroll
         for (int q = 0; q < GATHER_SLOTS; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0;
+        // Two incident elements at a time: both ids, node lists and
+        // coordinates are loaded together (three dependent rounds per pair
+        // instead of per element), then the two are integrated one after the
+        // other in ascending id.  The list is sorted, so a repeat is the same
+        // element again and is skipped.
         int64_t prev = -1;
-        for (int64_t k = k0; k < k1; ++k) {
-            const int64_t e = inc_list[k];
-            if (e == prev) continue;   // the list is sorted: a repeat is the same element again
-            prev = e;
-            Elem2D el;
-            if (!el.load_geom<MODE>(m, e, misc)) continue;
-            el.material();
-            // Only row node i's moments are formed (i is a run-time index);
-            // the accumulators are indexed by constants only and stay in
-            // registers.
-#pragma unroll 1
-            for (int i = 0; i < el.nn; ++i) {
-                if (nd_at(el, i) != a) continue;
-                RowMoments rm;
-                el.row_moments(i, rm);
-                const int4 mp = *reinterpret_cast<const int4*>(sc.map + e * 16 + i * 4);
-                const int32_t sl[4] = {mp.x, mp.y, mp.z, mp.w};
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (j >= el.nn) continue;
-                    double kb[4];
-                    el.row_block(rm, j, kb);
-#pragma unroll
-                    for (int q = 0; q < GATHER_SLOTS; ++q) {
-                        if (sl[j] == q) {
-                            acc[q][0] += kb[0];
-                            acc[q][1] += kb[1];
-                            acc[q][2] += kb[2];
-                            acc[q][3] += kb[3];
-                        }
-                    }
-                }
-            }
+        for (int64_t k = k0; k < k1; k += 2) {
+            const bool has1 = k + 1 < k1;
+            const int64_t e0 = inc_list[k];
+            const int64_t e1 = has1 ? (int64_t)inc_list[k + 1] : e0;
+            bool u0 = e0 != prev, u1 = has1 && e1 != e0;
+            prev = e1;
+            Elem2D el0, el1;
+            u0 = u0 && el0.load_nodes<MODE>(m, e0, misc);
+            u1 = u1 && el1.load_nodes<MODE>(m, e1, misc);
+            if (u0) el0.load_coords(m, e0);
+            if (u1) el1.load_coords(m, e1);
+            if (u0) gather_add_element(el0, e0, a, sc, acc);
+            if (u1) gather_add_element(el1, e1, a, sc, acc);
         }
 #pragma unroll
         for (int q = 0; q < GATHER_SLOTS; ++q) {
