@@ -305,6 +305,7 @@ __global__ __launch_bounds__(256) void k_scatter_2d(DevMesh m, Scatter sc, int32
 // of every local index of a in ascending order (row node, then column
 // node), the serial order.
 constexpr int GATHER_SLOTS = 9;
+constexpr int GATHER_BLOCK = 128;   // threads per block (LDS: 9 slots x 32 B each)
 
 // Node id i of an element, i a run-time value: a select chain, so that the
 // element's arrays keep constant indices and stay in registers (an indexed
@@ -314,12 +315,13 @@ __device__ __forceinline__ int32_t nd_at(const Elem2D& el, int i) {
 }
 
 // Integrates element e (loaded) for row node a and adds its blocks into the
-// row's register accumulators (block slots from the cached map).
+// row's block slots in LDS (slot from the cached map): one 32-byte
+// read-add-write per block, in the element's column order, so coincident
+// columns of one element add one after the other.
 __device__ __forceinline__ void gather_add_element(Elem2D& el, int64_t e, int64_t a, const Scatter& sc,
-                                                   double (&acc)[GATHER_SLOTS][4]) {
+                                                   double4* slots) {
     el.material();
-    // Only row node i's moments are formed (i is a run-time index); the
-    // accumulators are indexed by constants only and stay in registers.
+    // Only row node i's moments are formed (i is a run-time index).
 #pragma unroll 1
     for (int i = 0; i < el.nn; ++i) {
         if (nd_at(el, i) != a) continue;
@@ -332,22 +334,19 @@ __device__ __forceinline__ void gather_add_element(Elem2D& el, int64_t e, int64_
             if (j >= el.nn) continue;
             double kb[4];
             el.row_block(rm, j, kb);
-#pragma unroll
-            for (int q = 0; q < GATHER_SLOTS; ++q) {
-                if (sl[j] == q) {
-                    acc[q][0] += kb[0];
-                    acc[q][1] += kb[1];
-                    acc[q][2] += kb[2];
-                    acc[q][3] += kb[3];
-                }
-            }
+            double4* d = slots + (int64_t)sl[j] * GATHER_BLOCK;
+            const double4 o = *d;
+            *d = make_double4(o.x + kb[0], o.y + kb[1], o.z + kb[2], o.w + kb[3]);
         }
     }
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const int64_t* inc_off,
-                                                   const int32_t* inc_list, int32_t* misc) {
+__global__ __launch_bounds__(GATHER_BLOCK) void k_gather_2d(DevMesh m, Scatter sc, const int64_t* inc_off,
+                                                            const int32_t* inc_list, int32_t* misc) {
+    // Block slots of the thread's row, slot q at [q][thread]: consecutive
+    // threads, consecutive 32-byte blocks.
+    __shared__ double4 gslot[GATHER_SLOTS * GATHER_BLOCK];
     const int64_t a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (a >= m.n_nodes) return;
     const int64_t r0 = rp_at(sc.row_ptr, sc.i64, 2 * a);
@@ -355,34 +354,46 @@ __global__ __launch_bounds__(256) void k_gather_2d(DevMesh m, Scatter sc, const 
     const int64_t rowlen = r1 - r0;
     const int64_t k0 = inc_off[a], k1 = inc_off[a + 1];
     if (rowlen <= 2 * GATHER_SLOTS) {
-        double acc[GATHER_SLOTS][4];
+        double4* slots = gslot + threadIdx.x;
 #pragma unroll
-        for (int q = 0; q < GATHER_SLOTS; ++q) acc[q][0] = acc[q][1] = acc[q][2] = acc[q][3] = 0.0;
-        // Two incident elements at a time: both ids, node lists and
-        // coordinates are loaded together (three dependent rounds per pair
-        // instead of per element), then the two are integrated one after the
-        // other in ascending id.  The list is sorted, so a repeat is the same
-        // element again and is skipped.
+        for (int q = 0; q < GATHER_SLOTS; ++q) slots[q * GATHER_BLOCK] = make_double4(0.0, 0.0, 0.0, 0.0);
+        // Four incident elements at a time (every incident element of an
+        // interior quadrilateral row): their ids, node lists and coordinates
+        // are loaded together (three dependent rounds of loads per batch
+        // instead of per element), then integrated one after the other in
+        // ascending id.  The list is sorted, so a repeat is the same element
+        // again and is skipped.
         int64_t prev = -1;
-        for (int64_t k = k0; k < k1; k += 2) {
-            const bool has1 = k + 1 < k1;
-            const int64_t e0 = inc_list[k];
-            const int64_t e1 = has1 ? (int64_t)inc_list[k + 1] : e0;
-            bool u0 = e0 != prev, u1 = has1 && e1 != e0;
-            prev = e1;
-            Elem2D el0, el1;
-            u0 = u0 && el0.load_nodes<MODE>(m, e0, misc);
-            u1 = u1 && el1.load_nodes<MODE>(m, e1, misc);
-            if (u0) el0.load_coords(m, e0);
-            if (u1) el1.load_coords(m, e1);
-            if (u0) gather_add_element(el0, e0, a, sc, acc);
-            if (u1) gather_add_element(el1, e1, a, sc, acc);
+        for (int64_t k = k0; k < k1; k += 4) {
+            int64_t e[4];
+            bool u[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) e[b] = k + b < k1 ? (int64_t)inc_list[k + b] : -1;
+            u[0] = e[0] != prev;
+#pragma unroll
+            for (int b = 1; b < 4; ++b) u[b] = e[b] >= 0 && e[b] != e[b - 1];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) prev = e[b] >= 0 ? e[b] : prev;
+            Elem2D el0, el1, el2, el3;
+            u[0] = u[0] && el0.load_nodes<MODE>(m, e[0], misc);
+            u[1] = u[1] && el1.load_nodes<MODE>(m, e[1], misc);
+            u[2] = u[2] && el2.load_nodes<MODE>(m, e[2], misc);
+            u[3] = u[3] && el3.load_nodes<MODE>(m, e[3], misc);
+            if (u[0]) el0.load_coords(m, e[0]);
+            if (u[1]) el1.load_coords(m, e[1]);
+            if (u[2]) el2.load_coords(m, e[2]);
+            if (u[3]) el3.load_coords(m, e[3]);
+            if (u[0]) gather_add_element(el0, e[0], a, sc, slots);
+            if (u[1]) gather_add_element(el1, e[1], a, sc, slots);
+            if (u[2]) gather_add_element(el2, e[2], a, sc, slots);
+            if (u[3]) gather_add_element(el3, e[3], a, sc, slots);
         }
 #pragma unroll
         for (int q = 0; q < GATHER_SLOTS; ++q) {
             if (2 * q < rowlen) {
-                *reinterpret_cast<double2*>(sc.val + r0 + 2 * q) = make_double2(acc[q][0], acc[q][1]);
-                *reinterpret_cast<double2*>(sc.val + r1 + 2 * q) = make_double2(acc[q][2], acc[q][3]);
+                const double4 v = slots[q * GATHER_BLOCK];
+                *reinterpret_cast<double2*>(sc.val + r0 + 2 * q) = make_double2(v.x, v.y);
+                *reinterpret_cast<double2*>(sc.val + r1 + 2 * q) = make_double2(v.z, v.w);
             }
         }
         return;
@@ -596,10 +607,11 @@ void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic
 void launch_gather_2d(const DevMesh& m, int kind, const Scatter& sc, const int64_t* inc_off, const int32_t* inc_list,
                       int32_t* misc, hipStream_t s) {
     if (m.n_nodes <= 0) return;
-    const unsigned g = grid_for(m.n_nodes, 256);
-    if (kind == 0) hipLaunchKernelGGL(k_gather_2d<0>, dim3(g), dim3(256), 0, s, m, sc, inc_off, inc_list, misc);
-    else if (kind == 1) hipLaunchKernelGGL(k_gather_2d<1>, dim3(g), dim3(256), 0, s, m, sc, inc_off, inc_list, misc);
-    else hipLaunchKernelGGL(k_gather_2d<2>, dim3(g), dim3(256), 0, s, m, sc, inc_off, inc_list, misc);
+    const unsigned g = grid_for(m.n_nodes, GATHER_BLOCK);
+    const dim3 b(GATHER_BLOCK);
+    if (kind == 0) hipLaunchKernelGGL(k_gather_2d<0>, dim3(g), b, 0, s, m, sc, inc_off, inc_list, misc);
+    else if (kind == 1) hipLaunchKernelGGL(k_gather_2d<1>, dim3(g), b, 0, s, m, sc, inc_off, inc_list, misc);
+    else hipLaunchKernelGGL(k_gather_2d<2>, dim3(g), b, 0, s, m, sc, inc_off, inc_list, misc);
 }
 
 }  // namespace alls
