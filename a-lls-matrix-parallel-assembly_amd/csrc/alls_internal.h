@@ -78,13 +78,13 @@ struct ListPool {
     const int32_t* conn;       // 2D: the mesh's connectivity (column nodes of a group)
     int64_t n_nodes;           // home chunks
     int64_t cap_chunks;        // home + overflow capacity
-    int64_t n_elems;           // 2D: bound of a group's element id
+    int64_t n_elems;           // bound of a group's or segment's element id
     int32_t C;                 // entries per chunk (<= ALLS_MAX_CHUNK)
     int32_t B;                 // values per entry
     uint32_t cmagic;           // chunk_magic(C): s / C as a multiply-high
     int32_t long_row;          // reservations ending past this many entries record MISC_MAX_ROW
     int32_t ng;                // 4: Q4 groups (element id per group in geid); 3: T3 line groups
-                               // (below); 0: per-entry keys
+                               // (below); 20: Hex20 segments (below); 0: per-entry keys
     int32_t gpc;               // 2D: groups per chunk (C / ng)
     int32_t npe;               // 2D: conn stride
     int32_t _pad;
@@ -101,6 +101,16 @@ struct ListPool {
 // array); slot indices (ch * C + r) are the same as for the other layouts,
 // and list_val_slot / list_key map them to the line.
 constexpr int LINE3_SLOTS = 4;   // value-block slots per 128-byte line group
+
+// Hex20 segments (ng == 20).  Every reservation of a Hex20 mesh is 20 slots
+// per row node an element names (one strip per naming), each strip holds the
+// element's 20 column blocks in its node order, and chunks hold whole strips
+// (C a multiple of 20).  So slot idx = ch * C + r is column conn[e][idx % 20]
+// of element e = geid[idx / 20], and the lists carry one 4-byte element id
+// per strip instead of an 8-byte key per entry.  An id outside [0, n_elems)
+// (a strip of a failed call) reads as element 0, so that no read leaves the
+// connectivity.
+constexpr int SEG20 = 20;
 
 // Multiplier for s / C without a divide: floor(s * m / 2^32) with
 // m = ceil(2^32 / C) equals floor(s / C) for s < 2^32 / C (>= 262144 slots
@@ -126,6 +136,11 @@ struct Scatter {
 
 #ifdef __HIPCC__
 #define ALLS_DEV __device__ __forceinline__
+
+// Element id of a Hex20 strip, read as element 0 when out of range.
+ALLS_DEV int32_t seg_elem(const ListPool& p, int32_t e) {
+    return (uint32_t)e < (uint64_t)p.n_elems ? e : 0;
+}
 
 ALLS_DEV int64_t rp_at(const void* rp, int i64, int64_t i) {
     return i64 ? static_cast<const int64_t*>(rp)[i] : (int64_t) static_cast<const int32_t*>(rp)[i];
@@ -254,6 +269,11 @@ ALLS_DEV int64_t list_val_slot(const ListPool& p, int64_t idx) {
 }
 // (column, element) key of slot idx.
 ALLS_DEV int2 list_key(const ListPool& p, int64_t idx) {
+    if (p.ng == SEG20) {
+        const int64_t g = idx / SEG20;
+        const int32_t e = seg_elem(p, p.geid[g]);
+        return make_int2(p.conn[(int64_t)e * p.npe + (int)(idx - g * SEG20)], e);
+    }
     if (p.ng == 3) {
         const int64_t g = idx / 3;
         const int32_t* kb = reinterpret_cast<const int32_t*>(static_cast<const char*>(p.vals) + g * 128 + 96);

@@ -633,6 +633,7 @@ struct Pipeline {
 };
 
 static bool t3_line_groups(int64_t n_nodes);
+static bool hex20_segments(int C);
 
 // Chunk size, small-row lanes, scan tiles and list layout of a call.
 static void init_pipeline(Pipeline& pl, const alls_opts& o, const MeshInfo& mi, int64_t n_nodes) {
@@ -643,7 +644,18 @@ static void init_pipeline(Pipeline& pl, const alls_opts& o, const MeshInfo& mi, 
     if (mi.kind == 1) pl.C = std::max(3, pl.C / 3 * 3);   // whole T3 line groups per chunk
     pl.G = small_group(mi.kind);
     pl.ntiles = scan_tile_count(n_nodes);
-    pl.ng = mi.kind == 0 ? 4 : (mi.kind == 1 && t3_line_groups(n_nodes)) ? 3 : 0;
+    pl.ng = mi.kind == 0 ? 4 : (mi.kind == 1 && t3_line_groups(n_nodes)) ? 3
+          : (mi.kind == 3 && hex20_segments(pl.C)) ? SEG20 : 0;
+}
+
+// Hex20 lists carry one element id per 20-slot strip when chunks hold whole
+// strips; other chunk sizes key every entry.  ALLS_HEX20_KEYED=1 in the
+// environment forces per-entry keys (a diagnostic switch: the tests run both
+// layouts).
+static bool hex20_segments(int C) {
+    if (C % SEG20 != 0) return false;
+    const char* f = std::getenv("ALLS_HEX20_KEYED");
+    return !(f && f[0] && std::strcmp(f, "0") != 0);
 }
 
 // T3 line groups sort 32-bit keys (column << 5 | element-order slot), so
@@ -670,14 +682,16 @@ static alls_status alloc_lists(Workspace& ws, const DevMesh& dm, const MeshInfo&
     // Q4 lists carry one element id per group of 4 slots (geid); the
     // per-entry keys are written only for the rows the wave-per-row kernels
     // take.  T3 lists are line groups: each group of 3 slots in one 128-byte
-    // line of vals with its key block, and no key array.  Mixed and Hex20
-    // lists key every entry.  (T3 with element ids per group paid a
+    // line of vals with its key block, and no key array.  Hex20 lists carry
+    // one element id per 20-slot strip (geid) and no key array.  Mixed lists
+    // (and Hex20 lists with chunks that are not whole strips) key every
+    // entry.  (T3 with element ids per group paid a
     // scattered connectivity load per group in the fused pass, 1.34 ms
     // against 1.04 ms keyed at 1500x1500 with permuted node ids; the line
     // groups carry the columns themselves.)
     const int ng = pl.ng;
-    const size_t keys_bytes = ng == 3 ? 0 : (size_t)pl.cap_chunks * pl.C * sizeof(int2);
-    const size_t geid_bytes = ng == 4 ? (size_t)pl.cap_chunks * (pl.C / 4) * sizeof(int32_t) : 0;
+    const size_t keys_bytes = ng == 3 || ng == SEG20 ? 0 : (size_t)pl.cap_chunks * pl.C * sizeof(int2);
+    const size_t geid_bytes = ng == 4 || ng == SEG20 ? (size_t)pl.cap_chunks * (pl.C / ng) * sizeof(int32_t) : 0;
     const size_t vals_bytes = ng == 3 ? (size_t)pl.cap_chunks * (pl.C / 3) * 128
                                       : (size_t)pl.cap_chunks * pl.C * B * vsz;
     pl.list_bytes = (int64_t)((size_t)std::max<int64_t>(n_nodes, 1) * ALLS_DIR * sizeof(int32_t) + keys_bytes +
@@ -916,7 +930,7 @@ uint64_t front_key(const Workspace& ws, const DevMesh& dm, const MeshInfo& mi, c
         h = mixp(h, p);
     for (int64_t v : {dm.n_nodes, dm.n_elems, (int64_t)dm.npe, (int64_t)dm.dim, (int64_t)dm.uniform_type,
                       (int64_t)mi.kind, (int64_t)mi.f32, (int64_t)pl.C, pl.cap_chunks, pl.ntiles, (int64_t)pl.G,
-                      (int64_t)o.block_threads, (int64_t)o.grid_per_cu})
+                      (int64_t)pl.ng, (int64_t)o.block_threads, (int64_t)o.grid_per_cu})
         h = mix(h, (uint64_t)v);
     return mixd(mixd(mixd(h, dm.E), dm.nu), dm.thickness);
 }
@@ -931,7 +945,7 @@ uint64_t back_key(const Workspace& ws, const DevMesh& dm, const MeshInfo& mi, co
         h = mixp(h, p);
     for (int64_t v : {dm.n_nodes, (int64_t)mi.dpn, (int64_t)mi.f32, (int64_t)pl.C, pl.cap_chunks, pl.ntiles,
                       (int64_t)pl.G, (int64_t)pl.max_big, (int64_t)out->row_ptr_is_i64, (int64_t)dm.npe,
-                      (int64_t)mi.kind})
+                      (int64_t)mi.kind, (int64_t)pl.ng, dm.n_elems})
         h = mix(h, (uint64_t)v);
     return h;
 }

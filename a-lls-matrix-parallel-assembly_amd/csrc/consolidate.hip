@@ -1235,13 +1235,16 @@ __device__ __forceinline__ int64_t readlane64(int64_t x, int l) {
 // with the header and the row starts spread over lanes and read back with
 // readlane where used.  A load under a branch, or a uniform load (moved to a
 // scalar register at once), would make the compiler wait for every load
-// issued before it.
+// issued before it.  Segment lists (SG) have no keys: each lane's strip
+// element id is loaded with the row's header (it does not depend on it) and
+// its column is read from the connectivity, so the columns are as many
+// round trips away as keys are.
 // Rows outside that shape (a length that is not a multiple of SEG or above
 // the slots, a segment with mixed element ids or a repeated column, two
 // segments of one element, more than UMAX columns) are listed in `defer`
 // (count misc[MISC_DEFER]) for k_rows_merge_reg.
 constexpr int SEG_UMAX = 96;
-template <typename VT, typename RP, int K, int SEG, typename OT>
+template <typename VT, typename RP, int K, int SEG, typename OT, bool SG>
 __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* row_ptr, int32_t* col_ind,
                                                           OT* val, int32_t* defer) {
     constexpr int B = 9, D = 3;
@@ -1267,16 +1270,26 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
     // Header of the next row: lane l holds word l % ALLS_DIR.
     int32_t hv = p.hdr[min(a, p.n_nodes - 1) * ALLS_DIR + (lane & (ALLS_DIR - 1))];
     for (; a < p.n_nodes; a += gridDim.x) {
+        // SG: the strip element id of each register slot's entry in the
+        // home chunk (strip clamped into it; entries past it are read by the
+        // overflow path), issued before the header is waited for.
+        int32_t gw[K];
+        if constexpr (SG) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) gw[k] = p.geid[a * p.gpc + min(lane < LPS ? SPS * k + sl : 0, p.gpc - 1)];
+        }
         const int32_t hd = hv;   // this row's header
         const int32_t n = min(__builtin_amdgcn_readlane(hd, 0), ALLS_DIR * p.C);
         const int64_t an = a + gridDim.x;
         const int lim = min(n, p.C);
         int2 kk[K];
         VT vv[K][B];
+        if constexpr (!SG) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = LPS * k + lane;
-            kk[k] = p.keys[a * p.C + (lane < LPS && i < lim ? i : 0)];
+            for (int k = 0; k < K; ++k) {
+                const int i = LPS * k + lane;
+                kk[k] = p.keys[a * p.C + (lane < LPS && i < lim ? i : 0)];
+            }
         }
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -1284,6 +1297,16 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
             const VT* src = static_cast<const VT*>(p.vals) + (a * p.C + (lane < LPS && i < lim ? i : 0)) * B;
 #pragma unroll
             for (int b = 0; b < B; ++b) vv[k][b] = src[b];
+        }
+        if constexpr (SG) {
+            // After the values: these wait for the strip ids, the values
+            // only for the header.
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int i = LPS * k + lane;
+                const int32_t e = seg_elem(p, gw[k]);
+                kk[k] = make_int2(p.conn[(int64_t)e * p.npe + (lane < LPS && i < lim ? lane - SEG * sl : 0)], e);
+            }
         }
         // Starts of the 3 DOF rows (lane dr holds row dr's).
         const int64_t rv = (int64_t)row_ptr[a * D + min(lane, D - 1)];
@@ -1297,7 +1320,7 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
                 const int i = LPS * k + lane;
                 if (lane < LPS && i >= p.C && i < n) {
                     const int64_t ei = dir.entry(p, a, i);
-                    kk[k] = p.keys[ei];
+                    kk[k] = SG ? list_key(p, ei) : p.keys[ei];
                     const VT* src = static_cast<const VT*>(p.vals) + ei * B;
 #pragma unroll
                     for (int b = 0; b < B; ++b) vv[k][b] = src[b];
@@ -1318,28 +1341,38 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
             if (lane < LPS && i < n && lane % SEG == 0) seid[SPS * k + sl] = kk[k].y;
         }
         __syncthreads();
-        // 1. Hash set of the columns.
+        // 1. Hash set of the columns.  The K slots probe together: one LDS
+        // round trip per probe step for all of them.
         uint32_t slot[K];
+        bool pend[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int i = LPS * k + lane;
-            slot[k] = 0;
-            if (ok && lane < LPS && i < n) {
-                uint32_t h = col_hash(kk[k].x, shift);
-                for (int probe = 0; probe < TS; ++probe) {
-                    const int32_t old = atomicCAS(&table[h], -1, kk[k].x);
-                    if (old == -1 || old == kk[k].x) break;
-                    h = (h + 1) & (uint32_t)(TS - 1);
-                }
-                slot[k] = h;
+            pend[k] = ok && lane < LPS && i < n;
+            slot[k] = pend[k] ? col_hash(kk[k].x, shift) : 0u;
+        }
+        for (int probe = 0; probe < TS; ++probe) {
+            int32_t old[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) old[k] = pend[k] ? atomicCAS(&table[slot[k]], -1, kk[k].x) : -1;
+            bool more = false;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (pend[k] && old[k] != -1 && old[k] != kk[k].x) slot[k] = (slot[k] + 1) & (uint32_t)(TS - 1);
+                else pend[k] = false;
+                more |= pend[k];
             }
+            if (__ballot(more) == 0) break;
         }
         __syncthreads();
         // Unique index: occupied slots numbered in (j, lane) order.
+        int32_t tv[TS / 64];
+#pragma unroll
+        for (int j = 0; j < TS / 64; ++j) tv[j] = table[j * 64 + lane];
         int32_t u = 0;
 #pragma unroll
         for (int j = 0; j < TS / 64; ++j) {
-            const int32_t c = table[j * 64 + lane];
+            const int32_t c = tv[j];
             const uint64_t m = __ballot(c >= 0);
             const int32_t q = u + (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -1405,23 +1438,35 @@ __global__ __launch_bounds__(64, 4) void k_rows_merge_seg(ListPool p, const RP* 
                 }
             }
         }
-        // 4. Column ranks: lanes take q = lane and lane + 64.
+        // 4. Column ranks: lanes take q = lane, and q = lane + 64 when the
+        // row has more than 64 columns (uniform branch: most rows have fewer).
         {
             const int32_t c0 = lane < u ? ucol[lane] : 0x7fffffff;
-            const int32_t c1 = lane + 64 < u ? ucol[lane + 64] : 0x7fffffff;
-            int32_t r0 = 0, r1 = 0;
-            for (int k = 0; k < u16; k += 16) {
-                int4 w[4];
+            int32_t r0 = 0;
+            if (u <= 64) {
+                for (int k = 0; k < u16; k += 16) {
+                    int4 w[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) w[t] = *reinterpret_cast<const int4*>(ucol + k + 4 * t);
+                    for (int t = 0; t < 4; ++t) w[t] = *reinterpret_cast<const int4*>(ucol + k + 4 * t);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    r0 += (w[t].x < c0) + (w[t].y < c0) + (w[t].z < c0) + (w[t].w < c0);
-                    r1 += (w[t].x < c1) + (w[t].y < c1) + (w[t].z < c1) + (w[t].w < c1);
+                    for (int t = 0; t < 4; ++t) r0 += (w[t].x < c0) + (w[t].y < c0) + (w[t].z < c0) + (w[t].w < c0);
                 }
+            } else {
+                const int32_t c1 = lane + 64 < u ? ucol[lane + 64] : 0x7fffffff;
+                int32_t r1 = 0;
+                for (int k = 0; k < u16; k += 16) {
+                    int4 w[4];
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) w[t] = *reinterpret_cast<const int4*>(ucol + k + 4 * t);
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) {
+                        r0 += (w[t].x < c0) + (w[t].y < c0) + (w[t].z < c0) + (w[t].w < c0);
+                        r1 += (w[t].x < c1) + (w[t].y < c1) + (w[t].z < c1) + (w[t].w < c1);
+                    }
+                }
+                if (lane + 64 < u) rq[r1] = lane + 64;
             }
             if (lane < u) rq[r0] = lane;
-            if (lane + 64 < u) rq[r1] = lane + 64;
         }
         __syncthreads();
         // 5. A lane per column: its 3 x 3 block into the 3 DOF rows.
@@ -1521,7 +1566,7 @@ __global__ __launch_bounds__(64) void k_rows_hash_count(ListPool p, const int32_
 // unconditionally (lanes past a row read its slot 0) and every lane stores
 // the row's count, so that the number of memory operations between a load
 // and its use is fixed and the compiler waits for exactly what is used.
-template <int K>
+template <int K, bool SG>
 __global__ __launch_bounds__(64) void k_rows_count_long(ListPool p, int32_t* u, int32_t* defer) {
     constexpr int CAP = 64 * K;
     constexpr int TS = 2 * CAP;
@@ -1536,13 +1581,35 @@ __global__ __launch_bounds__(64) void k_rows_count_long(ListPool p, int32_t* u, 
     int64_t a = xcd_block(blockIdx.x, gridDim.x);
     int32_t ha = p.hdr[min(a, last) * ALLS_DIR + hl];       // header of row a
     int32_t hb = p.hdr[min(a + g, last) * ALLS_DIR + hl];   // header of row a + g
+    // Segment lists: each lane's strip element ids of a row's home chunk
+    // (no header needed), then the columns from the connectivity.  Entry i
+    // of the home chunk is column i % 20 of strip i / 20; lanes past the
+    // chunk read strip 0.
+    int32_t seg[K], sj[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = lane + 64 * k;
+        seg[k] = SG && i < p.C ? i / SEG20 : 0;
+        sj[k] = SG && i < p.C ? i - SEG20 * (i / SEG20) : 0;
+    }
+    int32_t gb[K];   // SG: strip element ids of row a + g
     int32_t kc[K];
     {
         const int lim = min(min(__builtin_amdgcn_readlane(ha, 0), ALLS_DIR * p.C), p.C);
+        if constexpr (SG) {
+            int32_t ga[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int i = lane + 64 * k;
-            kc[k] = p.keys[min(a, last) * p.C + (i < lim ? i : 0)].x;
+            for (int k = 0; k < K; ++k) ga[k] = p.geid[min(a, last) * p.gpc + seg[k]];
+#pragma unroll
+            for (int k = 0; k < K; ++k) gb[k] = p.geid[min(a + g, last) * p.gpc + seg[k]];
+#pragma unroll
+            for (int k = 0; k < K; ++k) kc[k] = p.conn[(int64_t)seg_elem(p, ga[k]) * p.npe + sj[k]];
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int i = lane + 64 * k;
+                kc[k] = p.keys[min(a, last) * p.C + (i < lim ? i : 0)].x;
+            }
         }
         // Settle the prologue: the loop's waits are then those of its own
         // iterations (the compiler merges the entry and back-edge states).
@@ -1553,8 +1620,15 @@ __global__ __launch_bounds__(64) void k_rows_count_long(ListPool p, int32_t* u, 
         const int32_t hd = ha;
         // Loads for the rows ahead.
         const int32_t hc = p.hdr[min(a + 2 * g, last) * ALLS_DIR + hl];
-        int32_t kn[K];
-        {
+        int32_t kn[K], gc[K];
+        if constexpr (SG) {
+            // Strip ids of row a + 2g, columns of row a + g (from its ids,
+            // loaded an iteration ago).
+#pragma unroll
+            for (int k = 0; k < K; ++k) gc[k] = p.geid[min(a + 2 * g, last) * p.gpc + seg[k]];
+#pragma unroll
+            for (int k = 0; k < K; ++k) kn[k] = p.conn[(int64_t)seg_elem(p, gb[k]) * p.npe + sj[k]];
+        } else {
             const int64_t b = min(a + g, last);
             const int lim = min(min(__builtin_amdgcn_readlane(hb, 0), ALLS_DIR * p.C), p.C);
 #pragma unroll
@@ -1573,7 +1647,7 @@ __global__ __launch_bounds__(64) void k_rows_count_long(ListPool p, int32_t* u, 
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const int i = lane + 64 * k;
-                if (i >= p.C && i < n) kc[k] = p.keys[dir.entry(p, a, i)].x;
+                if (i >= p.C && i < n) kc[k] = list_key(p, dir.entry(p, a, i)).x;
             }
             // Settle this rare path's loads here, so that the common path's
             // waits are not merged with it.
@@ -1604,7 +1678,10 @@ __global__ __launch_bounds__(64) void k_rows_count_long(ListPool p, int32_t* u, 
         u[a] = total;   // a deferred row's count is rewritten by the second launch
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < K; ++k) kc[k] = kn[k];
+        for (int k = 0; k < K; ++k) {
+            kc[k] = kn[k];
+            if constexpr (SG) gb[k] = gc[k];
+        }
         ha = hb;
         hb = hc;
     }
@@ -1810,8 +1887,12 @@ long_rows:
         // rows, so it wants several rows per wave: 65536 waves measured
         // 0.62 ms at 80^3 against 0.67 ms for 262144 and 1.18 ms for one
         // row per wave.
-        hipLaunchKernelGGL(k_rows_count_long<4>, dim3((unsigned)std::min<int64_t>(grid, 65536)), dim3(64), 0, s,
-                           p, u, bl);
+        if (p.ng == SEG20)
+            hipLaunchKernelGGL((k_rows_count_long<4, true>), dim3((unsigned)std::min<int64_t>(grid, 65536)), dim3(64),
+                               0, s, p, u, bl);
+        else
+            hipLaunchKernelGGL((k_rows_count_long<4, false>), dim3((unsigned)std::min<int64_t>(grid, 65536)), dim3(64),
+                               0, s, p, u, bl);
         hipLaunchKernelGGL(k_rows_hash_count, dim3(2048), dim3(64), (size_t)ts * sizeof(int32_t), s, p, bl, u, ts,
                            nullptr);
         return;
@@ -1833,15 +1914,22 @@ long_rows:
                 // (FP64) against 6.1-6.5 ms for 262144 waves looping over
                 // 8 rows each, whose next row waits for the last one's stores.
                 grid = std::max<int64_t>(p.n_nodes, 1);
+#define ALLS_SEG(KS)                                                                                        \
+    do {                                                                                                    \
+        if (p.ng == SEG20)                                                                                  \
+            hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, KS, 20, OT, true>), dim3((unsigned)grid), dim3(64), 0, \
+                               s, p, rp, ci, v, bl);                                                        \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, KS, 20, OT, false>), dim3((unsigned)grid), dim3(64), 0, \
+                               s, p, rp, ci, v, bl);                                                        \
+    } while (0)
                 if (ks == 2)
-                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 2, 20, OT>), dim3((unsigned)grid), dim3(64), 0, s, p,
-                                       rp, ci, v, bl);
+                    ALLS_SEG(2);
                 else if (ks == 3)
-                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 3, 20, OT>), dim3((unsigned)grid), dim3(64), 0, s, p,
-                                       rp, ci, v, bl);
+                    ALLS_SEG(3);
                 else
-                    hipLaunchKernelGGL((k_rows_merge_seg<VT, RP, 4, 20, OT>), dim3((unsigned)grid), dim3(64), 0, s, p,
-                                       rp, ci, v, bl);
+                    ALLS_SEG(4);
+#undef ALLS_SEG
                 hipLaunchKernelGGL((k_rows_merge_reg<B, VT, RP, 4, OT>), dim3(1024), dim3(64), 0, s, p, bl, rp, ci, v,
                                    MISC_DEFER);
                 return;
@@ -1911,6 +1999,7 @@ void launch_rows_fused(const ListPool& p, int G, int64_t* tstat, void* row_ptr, 
 }
 
 void launch_count(const ListPool& p, int G, int f32vals, int32_t* u, int32_t* big_list, hipStream_t s) {
+    if (p.n_nodes <= 0) return;   // no rows (the row kernels clamp row indices to the last row)
     if (p.B == 4)
         rows_dispatch<4, double, false, int32_t, double>(p, G, 0, big_list, u, big_list, nullptr, nullptr, nullptr, 2, s);
     else if (f32vals)
@@ -1956,6 +2045,7 @@ void launch_scan_apply(const int32_t* u, int64_t n, const int64_t* tiles, int dp
 void launch_write(const ListPool& p, int G, int f32vals, int dpn, int32_t max_big, int32_t* big_list,
                   const void* row_ptr, int i64, int32_t* col_ind, void* val, int val_f32, hipStream_t s) {
     (void)dpn;
+    if (p.n_nodes <= 0) return;   // no rows
     const int32_t mb = max_big;
     double* vd = static_cast<double*>(val);
     float* vf = static_cast<float*>(val);

@@ -389,6 +389,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                     b = ch < 0 ? -2 : ch * p.C + (pos - k0 * p.C);
                 }
                 rbase[lane] = b;
+                // Segment lists: the strip's element id, one store per row
+                // (a strip never crosses a chunk there, so b >= 0 unless
+                // the chunk could not be had, which fails the call).
+                if (p.ng == SEG20 && b >= 0) p.geid[b / SEG20] = (int32_t)e;
             }
         } else if (EMIT == 3) {
             // Element-order staging: row I's 20 blocks at entries e*400 + 20 I.
@@ -428,7 +432,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                         T* st = kst + ri * 180 + Jn * 9;
     #pragma unroll
                         for (int v = 0; v < 9; ++v) st[v] = kb[v / 3][v % 3];
-                        if (EMIT == 0) p.keys[id + Jn] = make_int2(nodes[Jn], (int32_t)e);
+                        if (EMIT == 0 && p.ng != SEG20) p.keys[id + Jn] = make_int2(nodes[Jn], (int32_t)e);
                         continue;
                     }
                     if (id >= 0) {
@@ -444,7 +448,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) ==
                     for (int r = 0; r < 3; ++r)
     #pragma unroll
                         for (int c = 0; c < 3; ++c) d[r * 3 + c] = kb[r][c];
-                    p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
+                    if (p.ng != SEG20) p.keys[id] = make_int2(nodes[Jn], (int32_t)e);
                 }
                 __syncthreads();
                 // The pass's row bases, read once (uniform per row), and all

@@ -470,3 +470,74 @@ def test_t3_batch_line_groups_equal_keyed_lists(ctx, monkeypatch):
     monkeypatch.delenv("ALLS_T3_KEYED")
     assert np.array_equal(ro, ro2) and np.array_equal(no, no2)
     assert same_csr(line, keyed)
+
+
+def _hex20_doubled(nx, ny, nz, seed, pick=None):
+    # Elements listed again as new elements (all of them, or `pick`).
+    m = alls.generate(alls.GEN_HEX20, nx, ny, nz, jitter=0.1, seed=seed, E_sigma=0.5)
+    c = m.conn.reshape(m.n_elems, 20)
+    idx = np.arange(m.n_elems) if pick is None else np.asarray(pick)
+    conn = np.concatenate([c, c[idx]])
+    E = None if m.E_per_elem is None else np.concatenate([m.E_per_elem, m.E_per_elem[idx]])
+    return alls.MeshArrays(m.coords, conn, 3, 20, m.uniform_type, None, m.E, E, m.nu, m.thickness)
+
+
+def _hex20_wide_row():
+    m = alls.generate(alls.GEN_HEX20, 3, 3, 3, jitter=0.1, seed=12)
+    conn = m.conn.reshape(m.n_elems, 20).copy()
+    x = int(conn[0, 0])
+    far = [e for e in range(m.n_elems) if x not in conn[e]][2::4][:6]
+    conn[far, 7] = x
+    return alls.MeshArrays(m.coords, conn, 3, 20, m.uniform_type, None, m.E, None, m.nu, m.thickness)
+
+
+HEX20_LAYOUT_CASES = [
+    ("brick", lambda: alls.generate(alls.GEN_HEX20, 5, 4, 3, jitter=0.15, seed=14, E_sigma=0.5), {}),
+    ("f32", lambda: alls.generate(alls.GEN_HEX20, 4, 3, 3, jitter=0.15, seed=15, E_sigma=0.5),
+     {"precision": alls.PREC_F32_MFMA}),
+    ("f32_float_values", lambda: alls.generate(alls.GEN_HEX20, 4, 3, 3, jitter=0.15, seed=16),
+     {"precision": alls.PREC_F32_MFMA, "val_f32": 1}),
+    ("overflow_chunks", lambda: alls.generate(alls.GEN_HEX20, 3, 3, 3, jitter=0.1, seed=8), {"chunk_slots": 40}),
+    ("rows_past_count_table", lambda: _hex20_doubled(2, 2, 2, 9), {}),
+    ("four_register_slots", lambda: _hex20_doubled(2, 2, 2, 11, [0, 5]), {}),
+    ("wide_row_deferred", _hex20_wide_row, {}),
+    ("repeated_node", lambda: repeated_node_mesh(*REPEATED[3])[1], {}),
+    ("single", lambda: alls.generate(alls.GEN_HEX20, 1, 1, 1), {}),
+]
+
+
+@pytest.mark.parametrize("name,make,opts", HEX20_LAYOUT_CASES, ids=[c[0] for c in HEX20_LAYOUT_CASES])
+def test_hex20_segment_lists_equal_keyed_lists(ctx, monkeypatch, name, make, opts):
+    # Hex20 lists carry one element id per 20-slot strip (columns from the
+    # connectivity); ALLS_HEX20_KEYED forces a key per entry.  Both give the
+    # same bits and match the oracle, through overflow chunks, the second
+    # count launch, four register slots, rows the segment merge defers, and
+    # an element that lists a node twice.
+    m = make()
+    print(f"hex20 layout case {name}", flush=True)
+    seg = ctx.assemble(m, **opts).download()
+    monkeypatch.setenv("ALLS_HEX20_KEYED", "1")
+    keyed = ctx.assemble(m, **opts).download()
+    monkeypatch.delenv("ALLS_HEX20_KEYED")
+    assert same_csr(seg, keyed)
+    ref = oracle.assemble(m)
+    if opts.get("precision", alls.PREC_F64) == alls.PREC_F64:
+        compare_csr(seg, ref)
+    else:
+        assert np.array_equal(seg[0], ref[0]) and np.array_equal(seg[1], ref[1])
+        assert np.linalg.norm(seg[2] - ref[2]) / np.linalg.norm(ref[2]) <= 1e-6
+
+
+def test_hex20_batch_segment_lists_equal_keyed_lists(ctx, monkeypatch):
+    meshes = [alls.generate(alls.GEN_HEX20, 2 + i % 3, 2, 1 + i % 2, jitter=0.1, seed=50 + i, E_sigma=0.5)
+              for i in range(5)]
+    out, ro, no = ctx.assemble_batch(meshes)
+    seg = out.download()
+    out.free()
+    monkeypatch.setenv("ALLS_HEX20_KEYED", "1")
+    out, ro2, no2 = ctx.assemble_batch(meshes)
+    keyed = out.download()
+    out.free()
+    monkeypatch.delenv("ALLS_HEX20_KEYED")
+    assert np.array_equal(ro, ro2) and np.array_equal(no, no2)
+    assert same_csr(seg, keyed)
