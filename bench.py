@@ -298,8 +298,8 @@ def hex20_compare(ctx, n, steps, peaks):
                      "ms_write": round(st["ms_write"], 3), "nnz": st["nnz"]}
         times = {"insert": st["ms_insert"], "count": st["ms_count"], "write": st["ms_write"]}
         t = "double" if name == "f64" else "float"
-        kernels = {"insert": f"k_hex20<{t}, 0>", "count": "k_rows_count_long<4>",
-                   "write": f"k_rows_merge_seg<{t}, int, 3, 20, double>"}
+        kernels = {"insert": f"k_hex20<{t}, 0>", "count": "k_rows_count_long<4, true>",
+                   "write": f"k_rows_merge_seg<{t}, int, 3, 20, double, true>"}
         pb = path_bytes(m, st["nnz"])
         res[name]["roofline"] = hbm_roofline(pb, times, kernels)
         res[name]["roofline"]["traffic"] = measured_traffic(res[name]["roofline"]["kernel"])[0]
