@@ -15,7 +15,9 @@
 // over the 10 upper-triangle 16x16 tiles of the symmetric 64x64 (padded)
 // product, 7 k-steps of 4 Gauss points.  M is kept in LDS as its 210 upper
 // 3x3 node blocks, K is formed from them and every row node's 20 entries are
-// appended to its list with one reservation atomic.
+// appended to its list with one reservation atomic.  With chunks of whole
+// 20-slot strips the lists carry the strip's element id once (its columns
+// are the element's nodes) instead of a key per entry.
 #include "alls_internal.h"
 
 namespace {

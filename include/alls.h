@@ -79,7 +79,9 @@ typedef struct {
 
 typedef struct {
     int32_t precision;        /* alls_precision (Hex20 only) */
-    int32_t chunk_slots;      /* entries per list-pool chunk; 0 => tuned default; at most 16384 */
+    int32_t chunk_slots;      /* entries per list-pool chunk; 0 => tuned default; at most 16384.
+                                 Hex20: a multiple of 20 lets the lists carry one element id per
+                                 20-entry strip; other sizes key every entry (same results) */
     int32_t block_threads;    /* threads per block of the insert kernel; 0 => tuned default */
     int32_t row_ptr_i64;      /* 1 => always int64 row_ptr; otherwise int64 only when nnz >= 2^31 */
     int32_t timing;           /* 1 => time each stage with hipEvents (alls_last_stats); 0 => the
