@@ -311,9 +311,10 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 // (2D) or one block (Hex20) per element.
 void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, int64_t max_blocks, hipStream_t s);
 void launch_insert_hex20(const DevMesh& m, const ListPool& p, int f32, int64_t max_blocks, hipStream_t s);
-// Pattern cache: build the scatter map; value-only and coloured scatter.
+// Pattern cache: build the scatter map (from the node -> element incidence);
+// value-only and coloured scatter.
 void launch_build_map(const DevMesh& m, int kind, const void* row_ptr, int i64, const int32_t* col_ind,
-                      int32_t* map, int32_t* misc, hipStream_t s);
+                      const int64_t* inc_off, const int32_t* inc_list, int32_t* map, int32_t* misc, hipStream_t s);
 // atomic = 1: FP64 atomics (all elements at once); 0: plain "+=" (one colour).
 void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic, int32_t* misc, hipStream_t s);
 void launch_gather_2d(const DevMesh& m, int kind, const Scatter& sc, const int64_t* inc_off, const int32_t* inc_list,

@@ -1,6 +1,7 @@
 // insert_2d.hip -- Q4 / T3 / mixed plane-stress kernels (gfx950):
 //   k_insert_2d  : element integration fused with A-LLS list insertion;
-//   k_map        : element -> CSR-slot map of a cached pattern;
+//   k_map        : element -> CSR-slot map of a cached pattern (2D; Hex20
+//                  patterns take k_map_rows, a node row at a time);
 //   k_scatter_2d : coloured (plain "+=") or FP64-atomic scatter through
 //                  that map;
 //   k_gather_2d  : value-only re-assembly, one thread per node row.
@@ -460,9 +461,10 @@ __global__ __launch_bounds__(GATHER_BLOCK) void k_gather_2d(DevMesh m, Scatter s
     }
 }
 
-// Rank of every (row node i, column node j) pair of element e in the cached
-// pattern, by binary search of column b*dpn in DOF row a*dpn (the search of
-// the reference's binary_search, Acc:214-222, done once per pattern).
+// 2D: rank of every (row node i, column node j) pair of element e in the
+// cached pattern, a thread per (element, row node), by binary search of
+// column b*dpn in DOF row a*dpn (the search of the reference's
+// binary_search, Acc:214-222, done once per pattern).
 template <int NN, typename RP>
 __global__ __launch_bounds__(256) void k_map(DevMesh m, const RP* rp, const int32_t* ci, int32_t* map,
                                              int32_t* misc) {
@@ -522,6 +524,77 @@ __global__ __launch_bounds__(256) void k_map(DevMesh m, const RP* rp, const int3
     }
 }
 
+// Hex20: the same ranks a node row at a time: G lanes take row a's
+// incident elements (inc_list: sorted by id, an element listed once per slot
+// that names a), NN lanes per element, lane j searching column node
+// conn[e][j] among the row's sorted columns (DOF row a * dpn of col_ind, a
+// few cache lines all lanes share).  The element's row slot i is found by a
+// ballot of its lanes for conn[e][j] == a; the r-th listing of an element
+// that names a twice takes the r-th such slot.  Writes map[e][i][j]; flags an
+// element that lists a node twice, and a column missing from the pattern.
+template <int NN, int G, typename RP>
+__global__ __launch_bounds__(256) void k_map_rows(DevMesh m, const RP* rp, const int32_t* ci, const int64_t* off,
+                                                  const int32_t* list, int32_t* map, int32_t* misc) {
+    constexpr int SPP = G / NN;    // elements per pass
+    constexpr int LPP = SPP * NN;  // lanes per pass (Hex20: 60 of 64)
+    static_assert(64 % G == 0 && SPP >= 1, "groups inside a wave, whole elements per pass");
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t a = gid / G;
+    const int l = (int)(gid % G);
+    const int wl = threadIdx.x & 63;
+    bool twice = false, missing = false;
+    if (a < m.n_nodes) {
+        const int dpn = m.dim;
+        const int64_t rs = (int64_t)rp[a * dpn];
+        const int32_t u = (int32_t)(((int64_t)rp[a * dpn + 1] - rs) / dpn);   // column nodes of row a
+        const int64_t lo = off[a], ne = off[a + 1] - lo;
+        const int sj = l % NN, sb = wl - sj;   // slot j of this lane, first wave lane of its element
+        for (int64_t s0 = 0; s0 < ne; s0 += SPP) {
+            const int64_t k = s0 + l / NN;
+            const bool act = l < LPP && k < ne;
+            int32_t e = 0, nn = 0, cj = -1;
+            if (act) {
+                e = list[lo + k];
+                const int kind = m.types ? (int)m.types[e] : m.uniform_type;
+                nn = kind == 0 ? 4 : kind == 1 ? 3 : 20;
+                if (sj < nn) cj = m.conn[(int64_t)e * m.npe + sj];
+            }
+            // The element's slots naming a, as bits relative to its first lane.
+            const uint64_t hit = __ballot(cj == (int32_t)a);
+            if (!act || sj >= nn) continue;
+            uint32_t mine = (uint32_t)(hit >> sb) & ((1u << NN) - 1u);
+            int r = 0;
+            for (int64_t q = k - 1; q >= 0 && list[lo + q] == e; --q) ++r;   // earlier listings of e in this row
+            twice |= r > 0;
+            for (int t = 0; t < r; ++t) mine &= mine - 1u;
+            if (mine == 0u) {   // the incidence list and the connectivity disagree: never
+                missing = true;
+                continue;
+            }
+            const int i = __builtin_ctz(mine);
+            const int32_t tgt = cj * dpn;
+            int32_t lo2 = 0, hi2 = u - 1, res = -1;
+            while (lo2 <= hi2) {
+                const int32_t mid = (lo2 + hi2) >> 1;
+                const int32_t v = ci[rs + (int64_t)mid * dpn];
+                if (v == tgt) {
+                    res = mid;
+                    break;
+                }
+                if (v < tgt) lo2 = mid + 1;
+                else hi2 = mid - 1;
+            }
+            if (res < 0) {
+                missing = true;
+                res = 0;
+            }
+            map[(int64_t)e * NN * NN + i * NN + sj] = res;
+        }
+    }
+    if (twice) st_agent(&misc[MISC_ERR + ERR_REPEATED_NODE], 1);
+    if (missing) st_agent(&misc[MISC_ERR + ERR_INVALID_MESH], 1);
+}
+
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -573,21 +646,30 @@ void launch_insert_2d(const DevMesh& m, const ListPool& p, int mode, int block, 
 }
 
 void launch_build_map(const DevMesh& m, int kind, const void* row_ptr, int i64, const int32_t* col_ind,
-                      int32_t* map, int32_t* misc, hipStream_t s) {
-    if (m.n_elems <= 0) return;
-    const int npe_map = kind == 3 ? 20 : 4;
-    const unsigned g = grid_for(m.n_elems * npe_map, 256);
-#define ALLS_MAP(NN, RP)                                                                                \
-    hipLaunchKernelGGL((k_map<NN, RP>), dim3(g), dim3(256), 0, s, m, static_cast<const RP*>(row_ptr), \
-                       col_ind, map, misc)
-    if (npe_map == 20) {
-        if (i64) ALLS_MAP(20, int64_t);
-        else ALLS_MAP(20, int32_t);
-    } else {
-        if (i64) ALLS_MAP(4, int64_t);
-        else ALLS_MAP(4, int32_t);
+                      const int64_t* inc_off, const int32_t* inc_list, int32_t* map, int32_t* misc, hipStream_t s) {
+    if (m.n_elems <= 0 || m.n_nodes <= 0) return;
+    if (kind == 3) {
+        // Hex20: a wave per node row, three elements of 20 lanes per pass.
+        // 80^3 pattern build 18.0-18.5 -> 16.1-16.2 ms against a thread per
+        // (element, row node).  2D keeps the thread per (element, row node):
+        // row at a time was the same on Q4 2000^2 and 0.2 ms slower on the
+        // permuted-id T3 and mixed 1500^2 meshes (scattered elements per row).
+        const unsigned g = grid_for(m.n_nodes * 64, 256);
+        if (i64)
+            hipLaunchKernelGGL((k_map_rows<20, 64, int64_t>), dim3(g), dim3(256), 0, s, m,
+                               static_cast<const int64_t*>(row_ptr), col_ind, inc_off, inc_list, map, misc);
+        else
+            hipLaunchKernelGGL((k_map_rows<20, 64, int32_t>), dim3(g), dim3(256), 0, s, m,
+                               static_cast<const int32_t*>(row_ptr), col_ind, inc_off, inc_list, map, misc);
+        return;
     }
-#undef ALLS_MAP
+    const unsigned g = grid_for(m.n_elems * 4, 256);
+    if (i64)
+        hipLaunchKernelGGL((k_map<4, int64_t>), dim3(g), dim3(256), 0, s, m, static_cast<const int64_t*>(row_ptr),
+                           col_ind, map, misc);
+    else
+        hipLaunchKernelGGL((k_map<4, int32_t>), dim3(g), dim3(256), 0, s, m, static_cast<const int32_t*>(row_ptr),
+                           col_ind, map, misc);
 }
 
 void launch_scatter_2d(const DevMesh& m, int kind, const Scatter& sc, int atomic, int32_t* misc, hipStream_t s) {

@@ -235,9 +235,10 @@ alls_status pattern_build(Workspace& ws, const alls_mesh* mesh, const alls_opts&
     const int npm = mi.kind == 3 ? 20 : 4;
     if (p->map.ensure(std::max<int64_t>(mesh->n_elems, 1) * npm * npm * sizeof(int32_t)) != hipSuccess)
         return fail(hip_fail(hipErrorOutOfMemory, "alloc scatter map", err));
+    if ((rc = build_incidence(p, s, err)) != ALLS_OK) return fail(rc);
     launch_fill_i32(ws.misc.as<int32_t>(), MISC_SIZE, 0, s);
-    launch_build_map(p->dm, mi.kind, p->row_ptr.p, p->i64, p->col_ind.as<int32_t>(), p->map.as<int32_t>(),
-                     ws.misc.as<int32_t>(), s);
+    launch_build_map(p->dm, mi.kind, p->row_ptr.p, p->i64, p->col_ind.as<int32_t>(), p->inc_off.as<int64_t>(),
+                     p->inc_list.as<int32_t>(), p->map.as<int32_t>(), ws.misc.as<int32_t>(), s);
     int32_t flags[MISC_SIZE];
     if (hipMemcpyAsync(flags, ws.misc.p, sizeof(flags), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
@@ -247,7 +248,6 @@ alls_status pattern_build(Workspace& ws, const alls_mesh* mesh, const alls_opts&
         return fail(ALLS_E_HIP);
     }
     p->repeated_node = flags[MISC_ERR + ERR_REPEATED_NODE] != 0;
-    if ((rc = build_incidence(p, s, err)) != ALLS_OK) return fail(rc);
     if (mi.kind == 3 && (rc = hex20_row_limit(p, ws.misc.as<int32_t>(), s, err)) != ALLS_OK) return fail(rc);
     if (hipStreamSynchronize(s) != hipSuccess) return fail(hip_fail(hipGetLastError(), "sync pattern", err));
     *out_pat = p;
