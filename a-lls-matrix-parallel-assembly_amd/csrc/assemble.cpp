@@ -402,8 +402,6 @@ alls_status hip_fail(hipError_t e, const char* what, std::string* err) {
         if (e_ != hipSuccess) return hip_fail(e_, what, err); \
     } while (0)
 
-static int kind_nodes(int t) { return t == ALLS_Q4 ? 4 : t == ALLS_T3 ? 3 : t == ALLS_HEX20 ? 20 : -1; }
-
 alls_status classify_mesh(const alls_mesh* m, const alls_opts& o, MeshInfo* mi, std::string* err) {
     if (!m || !mi) {
         if (err) *err = "null mesh";
