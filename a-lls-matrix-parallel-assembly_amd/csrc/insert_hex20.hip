@@ -18,7 +18,7 @@
 // appended to its list with one reservation atomic.  With chunks of whole
 // 20-slot strips the lists carry the strip's element id once (its columns
 // are the element's nodes) instead of a key per entry.
-#include "alls_internal.h"
+#include "hex20_block.h"
 
 namespace {
 
@@ -51,7 +51,6 @@ constexpr int HEX20_STAGE_ROWS_F64 = 2;
 constexpr int LDS_ELEMS = MB_DUMP + 64 > 64 * XT ? MB_DUMP + 64 : 64 * XT;
 
 // Index of node pair (I, J), I <= J, in the packed upper triangle.
-__device__ __forceinline__ int pair_index(int I, int J) { return I * (39 - I) / 2 + J; }
 
 // dN/d(xi,eta,zeta) of node n at a Gauss point (Hex:95-128).
 constexpr void hex20_dshape(int n, double xi, double eta, double zeta, double& dx, double& de, double& dz) {
@@ -141,25 +140,6 @@ template <> struct OutRow<double> {
 template <> struct OutRow<float> {
     static __device__ __forceinline__ int row(int lk, int q) { return 4 * lk + q; }
 };
-
-// Block (I, J) of the element matrix from the packed upper blocks of M
-// (block (J, I) transposed when I > J: M is symmetric):
-//   K_IJ = lam * M_IJ + mu * M_IJ^T + mu * tr(M_IJ) * I3.
-template <typename T>
-__device__ __forceinline__ void k_block(const T* Mb, int I, int Jn, T lam, T mu, T (&kb)[3][3]) {
-    const bool up = I <= Jn;
-    const int base = 9 * (up ? pair_index(I, Jn) : pair_index(Jn, I));
-    T mb[3][3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) mb[r][c] = Mb[base + (up ? 3 * r + c : 3 * c + r)];
-    const T tr = mb[0][0] + mb[1][1] + mb[2][2];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) kb[r][c] = lam * mb[r][c] + mu * mb[c][r] + (r == c ? mu * tr : (T)0);
-}
 
 // Rows of list values staged per pass of the insertion (LDS: rows * 720 B
 // in FP32, rows * 1440 B in FP64).
