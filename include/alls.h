@@ -169,7 +169,7 @@ alls_status alls_assemble(alls_ctx* ctx, const alls_mesh* mesh, const alls_opts*
  * consolidation arrays live in a fixed device array of the library, 128 MiB
  * per GPU, held until the process ends; a mesh whose lists need more
  * returns ALLS_E_UNSUPPORTED (structured meshes of up to about 130k Q4
- * elements, 174k triangles, or 1,700 Hex20 elements in FP64 and 3,300 in
+ * elements, 174k triangles, or 1,900 Hex20 elements in FP64 and 4,200 in
  * FP32-MFMA fit).  The mesh arrays must be device memory (mem =
  * ALLS_MEM_DEVICE, 16-byte aligned); opts->autotune must be 0.  `out` holds
  * the caller's device buffers: row_ptr (cap_rows entries, int64 when
