@@ -67,3 +67,37 @@ def test_random_meshes_in_batches(ctx):
             for a, b in zip(p, alone.download()):
                 assert np.array_equal(a, b)
             alone.free()
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_mesh_chunk_sizes_and_layouts(ctx, monkeypatch, seed):
+    # Random chunk sizes (small ones give every row overflow chunks; Hex20
+    # takes strips of 20 slots when the size is a multiple of 20, keyed
+    # entries otherwise) and both list layouts of each kind (T3 / mixed line
+    # groups or keys, Hex20 strips or keys): every variant gives the same
+    # bits, and the oracle's matrix.
+    m = random_mesh(3000 + seed)
+    rng = np.random.default_rng(seed)
+    if m.dim == 3:
+        sizes = [0, int(rng.choice([20, 40, 60, 80])), int(rng.integers(21, 140))]
+        env = "ALLS_HEX20_KEYED"
+    else:
+        sizes = [0, int(rng.integers(4, 13)), int(rng.integers(13, 64))]
+        env = "ALLS_T3_KEYED"
+    print(f"random mesh {3000 + seed}: dim {m.dim}, {m.n_elems} elements, chunk sizes {sizes}", flush=True)
+    ref = oracle.assemble(m)
+    first = None
+    for keyed in (False, True):
+        if keyed:
+            monkeypatch.setenv(env, "1")
+        for c in sizes:
+            out = ctx.assemble(m, chunk_slots=c)
+            got = out.download()
+            out.free()
+            if first is None:
+                first = got
+                compare_csr(got, ref)
+            else:
+                for a, b in zip(got, first):
+                    assert np.array_equal(a, b), (keyed, c)
+    monkeypatch.delenv(env)
