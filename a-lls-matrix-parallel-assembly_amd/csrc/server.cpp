@@ -6,10 +6,12 @@
 // HBM footprint fits the remaining budget, so requests that arrive while
 // others run start as soon as a slot and memory free up, and their kernels
 // overlap on the device.  The general workers take the queue in arrival
-// order; with two or more workers the last one is an express lane for small
-// requests, which may pass a larger request at the head of the queue while
-// that head has waited less than an aging window (so the head is never
-// starved).  Requests that carry a mesh key can keep that mesh's pattern on
+// order, with at most one heavy request (a plain assembly of more than about
+// 8 ms) on the device at a time: a second general worker takes a lighter
+// request from behind a heavy head.  With two or more workers the last one
+// is an express lane for small requests, which may pass a larger request at
+// the head of the queue while that head has waited less than an aging
+// window (so the head is never starved).  Requests that carry a mesh key can keep that mesh's pattern on
 // the device, so a repeat request with a new E field runs only the
 // value-only path.
 //
@@ -61,6 +63,7 @@ struct Request {
     bool built = false;
     int batch_size = 1;
     int running_at_admit = 0;       // requests already on the device when it was admitted
+    bool heavy = false;             // counted in heavy_running while it runs
     uint64_t admit_index = 0;       // the server's admission count including it
     alls_csr out{};
     std::string err;
@@ -194,6 +197,7 @@ struct alls_server {
     // with the levels measured on the served mix.
     double cost[2][3] = {{5.0, 6.5, 2.5}, {40.0, 150.0, 36.0}};
     int running = 0;                 // requests on the device
+    int heavy_running = 0;           // heavy requests on the device (see pick)
     uint64_t admits = 0;             // admissions so far
 
     enum { PATH_PLAIN = 0, PATH_BUILD = 1, PATH_HIT = 2 };
@@ -256,6 +260,13 @@ struct alls_server {
     static constexpr double kAgeMs = 40.0;
     bool small(const Request& q) const {
         return (double)q.mesh.n_elems * cost[dim_of(q)][PATH_PLAIN] * 1e-6 <= kSmallMs;
+    }
+    // Heavy: a plain assembly of more than kHeavyMs at the measured cost
+    // (Hex20 from about 2e5 elements; no 2D mesh of up to 1e6).  One at a
+    // time on the device (see pick).
+    static constexpr double kHeavyMs = 8.0;
+    bool heavy(const Request& q) const {
+        return (double)q.mesh.n_elems * cost[dim_of(q)][PATH_PLAIN] * 1e-6 > kHeavyMs;
     }
     // Whether queued request q would run the plain path if it were the head.
     bool runs_plain(const Request& q) const {
@@ -326,7 +337,8 @@ struct alls_server {
     }
 
     // Index of the queued request worker `slot` may start, or -1.  General
-    // workers take the head.  The express worker takes only small requests:
+    // workers take the head, unless it is heavy and another heavy request is
+    // on the device (see below).  The express worker takes only small requests:
     // the head if it is small, else the first small one among the next 64
     // while the head has waited less than kAgeMs (after that it waits, so
     // that memory frees up for the head).  Letting it take large heads as
@@ -335,7 +347,24 @@ struct alls_server {
     int pick(int slot) const {
         if (queue.empty()) return -1;
         const bool express = cfg.n_streams >= 2 && slot == cfg.n_streams - 1;
-        if (!express || small(*queue.front())) return 0;
+        if (!express) {
+            // General workers: a heavy head waits while another heavy request
+            // runs; meanwhile a lighter request behind it may go, while the
+            // head has waited less than kAgeMs.  On the served mix (one MI355X,
+            // same arrival rate) 3 workers without this rule had half the p99
+            // of 2 (33-37 against 73-80 ms) in most runs, but in 2 of 12 the
+            // queue diverged (p99 0.66 s and 5.4 s with 4 workers: two large
+            // Hex20 requests at once hold two list pools of tens of GB, push the
+            // cached patterns out and share the device); with it, 41-57 ms in
+            // all 17 runs.
+            if (heavy_running == 0 || !heavy(*queue.front())) return 0;
+            if (ms_between(queue.front()->t_submit, Clock::now()) >= kAgeMs) return -1;
+            const int lim = (int)std::min<size_t>(queue.size(), 64);
+            for (int i = 1; i < lim; ++i)
+                if (!heavy(*queue[i])) return i;
+            return -1;
+        }
+        if (small(*queue.front())) return 0;
         if (ms_between(queue.front()->t_submit, Clock::now()) >= kAgeMs) return -1;   // the head's turn
         const int lim = (int)std::min<size_t>(queue.size(), 64);
         for (int i = 1; i < lim; ++i)
@@ -386,6 +415,8 @@ struct alls_server {
                             r = q;
                             queue.erase(queue.begin() + qi);
                             r->running_at_admit = running++;
+                            r->heavy = heavy(*r);
+                            heavy_running += r->heavy ? 1 : 0;
                             r->admit_index = ++admits;
                             if (qi > 0) counts.passed++;
                             if (r->key) note_seen(r->key, dim_of(*r));
@@ -501,6 +532,7 @@ struct alls_server {
                 r->status = rc;
                 if (batch.empty()) observe(*r, hit ? PATH_HIT : build ? PATH_BUILD : PATH_PLAIN);
                 --running;
+                heavy_running -= r->heavy ? 1 : 0;
                 r->cache_hit = hit;
                 r->built = build && rc == ALLS_OK;
                 r->done = true;
@@ -545,7 +577,7 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
     if (hipSetDevice(cfg->device) != hipSuccess) return ALLS_E_HIP;
     auto* s = new alls_server();
     s->cfg = *cfg;
-    const int n = cfg->n_streams <= 0 ? 2 : std::min(cfg->n_streams, 16);
+    const int n = cfg->n_streams <= 0 ? 3 : std::min(cfg->n_streams, 16);
     s->cfg.n_streams = n;
     if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 1 << 20;   // bounded by the budget
     size_t free_b = 0, total_b = 0;

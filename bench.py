@@ -490,11 +490,12 @@ def serve_costs_from(res, extra, n_q4):
     return c
 
 
-def served_mix(n_requests, costs, streams=2):
+def served_mix(n_requests, costs, streams=3):
     """Poisson mix of Q4/T3/Hex20 requests (1e3-1e6 elements, 60% mesh reuse).
     The arrival rate puts the offered load at 75% of one stream's measured
     service rate: a separate sample of 200 requests from the same
-    distribution is served one at a time first.  The per-element cost model
+    distribution is served one at a time first, by a server with one worker
+    (so the rate does not depend on the served configuration).  The per-element cost model
     (`costs`, from this run's large-mesh measurements) is reported beside it;
     it underestimates small and concurrent requests."""
     import alls
@@ -503,7 +504,7 @@ def served_mix(n_requests, costs, streams=2):
     meshes, reqs = serve_mix.build_requests(n_requests, progress=log)
     _, cal = serve_mix.build_requests(200, seed=0xA11500C6)
     t_gen = time.perf_counter() - t0
-    with alls.Server(n_streams=streams) as srv:
+    with alls.Server(n_streams=1) as srv:
         per_req_ms = serve_mix.one_stream_ms(srv, cal)
     rate = 0.75 * 1e3 / per_req_ms
     busy_s = serve_mix.offered_busy_s(reqs, costs)
@@ -541,7 +542,7 @@ def main():
     ap.add_argument("--extras", default="coloring,autotune,batch,c3,hex20,serve",
                     help="comma list of secondary measurements to run (or 'none')")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-streams", type=int, default=2, help="server workers (the last is the express lane)")
+    ap.add_argument("--serve-streams", type=int, default=3, help="server workers (the last is the express lane)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="end with one device-to-device copy of BYTES (a copy kernel of known size, the "
                          "scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")

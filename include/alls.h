@@ -274,8 +274,9 @@ typedef enum { ALLS_CACHE_COST = 0, ALLS_CACHE_EAGER = 1 } alls_cache_policy;
 
 typedef struct {
     int32_t device;
-    int32_t n_streams;        /* concurrent in-flight requests; 0 => 2 (more streams share the
-                                 device's 4 hardware queues and lose to allocation stalls) */
+    int32_t n_streams;        /* concurrent in-flight requests; 0 => 3: two general workers (one
+                                 heavy request, over about 8 ms, on the device at a time) and,
+                                 with 2 or more, the last an express lane for small requests */
     int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM */
     int32_t pattern_cache_entries;
     int32_t cache_policy;     /* alls_cache_policy */
