@@ -350,7 +350,8 @@ struct alls_server {
         if (!express) {
             // General workers: a heavy head waits while another heavy request
             // runs; meanwhile a lighter request behind it may go, while the
-            // head has waited less than kAgeMs.  On the served mix (one MI355X,
+            // head has waited less than kAgeMs.  (It matters with 3 or more
+            // workers, two of them general.)  On the served mix (one MI355X,
             // same arrival rate) 3 workers without this rule had half the p99
             // of 2 (33-37 against 73-80 ms) in most runs, but in 2 of 12 the
             // queue diverged (p99 0.66 s and 5.4 s with 4 workers: two large
@@ -577,7 +578,7 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
     if (hipSetDevice(cfg->device) != hipSuccess) return ALLS_E_HIP;
     auto* s = new alls_server();
     s->cfg = *cfg;
-    const int n = cfg->n_streams <= 0 ? 3 : std::min(cfg->n_streams, 16);
+    const int n = cfg->n_streams <= 0 ? 2 : std::min(cfg->n_streams, 16);
     s->cfg.n_streams = n;
     if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 1 << 20;   // bounded by the budget
     size_t free_b = 0, total_b = 0;

@@ -274,9 +274,10 @@ typedef enum { ALLS_CACHE_COST = 0, ALLS_CACHE_EAGER = 1 } alls_cache_policy;
 
 typedef struct {
     int32_t device;
-    int32_t n_streams;        /* concurrent in-flight requests; 0 => 3: two general workers (one
-                                 heavy request, over about 8 ms, on the device at a time) and,
-                                 with 2 or more, the last an express lane for small requests */
+    int32_t n_streams;        /* concurrent in-flight requests; 0 => 2.  With 2 or more the last
+                                 is an express lane for small requests; the others take the queue
+                                 in order, one heavy request (over about 8 ms) at a time.  3 gives
+                                 lower latency when HBM is plentiful, 2 keeps more patterns cached */
     int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM */
     int32_t pattern_cache_entries;
     int32_t cache_policy;     /* alls_cache_policy */
