@@ -551,6 +551,15 @@ class _Hip:
         self.L.hipFree.argtypes = [vp]
         self.L.hipDeviceSynchronize.argtypes = []
         self.L.hipSetDevice.argtypes = [C.c_int]
+        self.L.hipMemGetInfo.argtypes = [P(C.c_size_t), P(C.c_size_t)]
+
+    def mem_info(self):
+        """(free, total) bytes of the current device's HBM."""
+        f, t = C.c_size_t(), C.c_size_t()
+        r = self.L.hipMemGetInfo(C.byref(f), C.byref(t))
+        if r != 0:
+            raise RuntimeError(f"hipMemGetInfo failed: {r}")
+        return f.value, t.value
 
     def malloc(self, n):
         p = vp()

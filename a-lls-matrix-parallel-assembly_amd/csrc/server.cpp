@@ -581,6 +581,11 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
     const int n = cfg->n_streams <= 0 ? 2 : std::min(cfg->n_streams, 16);
     s->cfg.n_streams = n;
     if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 1 << 20;   // bounded by the budget
+    // Idle blocks in the process's allocation cache (buffers of destroyed
+    // contexts and freed outputs, up to a quarter of HBM) go back to the
+    // device first, so the default budget is taken from memory that is really
+    // free rather than shrunk by blocks nothing uses.
+    dev_trim();
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
         delete s;

@@ -637,7 +637,8 @@ def main():
             extra["hex20"] = hex20_compare(ctx, a.hex_n, 3, peaks)
         except Exception as e:  # report, do not hide the main line
             extra["hex20"] = {"error": str(e)}
-    ctx.close()   # frees the context's scratch before the server takes its HBM budget
+    ctx.close()   # its buffers go to the allocation cache, which the server hands back to the
+                  # device before it takes its HBM budget
     if "serve" in ex and a.serve_requests > 0:
         try:
             log(f"served mix: {a.serve_requests} requests")

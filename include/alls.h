@@ -277,8 +277,9 @@ typedef struct {
     int32_t n_streams;        /* concurrent in-flight requests; 0 => 2.  With 2 or more the last
                                  is an express lane for small requests; the others take the queue
                                  in order, one heavy request (over about 8 ms) at a time.  3 gives
-                                 lower latency when HBM is plentiful, 2 keeps more patterns cached */
-    int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM */
+                                 lower latency, 2 keeps more patterns cached */
+    int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM, measured after the
+                                 process's idle cached blocks are returned to the device */
     int32_t pattern_cache_entries;
     int32_t cache_policy;     /* alls_cache_policy */
 } alls_server_config;

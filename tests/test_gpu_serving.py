@@ -143,3 +143,23 @@ def test_cache_hit_requires_the_same_material_constants():
     ref = oracle.assemble(m_nu)
     compare_csr(got, ref)
     compare_csr(again, ref)
+
+
+def test_server_returns_idle_cached_blocks_before_taking_its_budget():
+    """A destroyed context's buffers stay in the process's allocation cache
+    (idle, still held on the device).  A server created afterwards hands them
+    back to the device before it sizes its default budget (55% of free HBM),
+    so they are not lost to it."""
+    h = alls.hip()
+    m = alls.generate(alls.GEN_Q4, 500, 500, jitter=0.2, seed=7)
+    c = alls.Context(0)
+    out = c.assemble(m)
+    held = c.stats()["scratch_bytes"]
+    assert held > 64 << 20
+    out.free()
+    c.close()
+    free_cached, _ = h.mem_info()     # the blocks are idle but still held
+    with alls.Server(n_streams=1):
+        pass
+    free_after, _ = h.mem_info()
+    assert free_after - free_cached >= held // 2, (free_cached, free_after, held)
