@@ -641,7 +641,7 @@ class DeviceMesh:
 class Server:
     """Continuous-batching assembly service (alls_server_*)."""
 
-    def __init__(self, device=0, n_streams=2, hbm_budget_bytes=0, pattern_cache_entries=0, cache_policy=CACHE_COST):
+    def __init__(self, device=0, n_streams=3, hbm_budget_bytes=0, pattern_cache_entries=0, cache_policy=CACHE_COST):
         cfg = ServerConfig()
         cfg.device, cfg.n_streams = device, n_streams
         cfg.hbm_budget_bytes, cfg.pattern_cache_entries = hbm_budget_bytes, pattern_cache_entries

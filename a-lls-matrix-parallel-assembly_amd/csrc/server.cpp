@@ -523,7 +523,9 @@ struct alls_server {
             // A workspace grown past an eighth of the budget (the list pool
             // of a large Hex20 request) is given back: held, it would push
             // cached patterns out, and the arena carves it again quickly.
-            if ((int64_t)w.device_bytes() > budget / 8) w.release_device();
+            // With two general workers (3 or more in all) two such pools can
+            // be held, so the limit is a sixteenth.
+            if ((int64_t)w.device_bytes() > budget / (cfg.n_streams >= 3 ? 16 : 8)) w.release_device();
             {
                 std::lock_guard<std::mutex> lk(mu);
                 ws_bytes[slot] = (int64_t)w.device_bytes();
@@ -578,7 +580,7 @@ alls_status alls_server_create(const alls_server_config* cfg, alls_server** out)
     if (hipSetDevice(cfg->device) != hipSuccess) return ALLS_E_HIP;
     auto* s = new alls_server();
     s->cfg = *cfg;
-    const int n = cfg->n_streams <= 0 ? 2 : std::min(cfg->n_streams, 16);
+    const int n = cfg->n_streams <= 0 ? 3 : std::min(cfg->n_streams, 16);
     s->cfg.n_streams = n;
     if (s->cfg.pattern_cache_entries <= 0) s->cfg.pattern_cache_entries = 1 << 20;   // bounded by the budget
     // Idle blocks in the process's allocation cache (buffers of destroyed

@@ -490,7 +490,7 @@ def serve_costs_from(res, extra, n_q4):
     return c
 
 
-def served_mix(n_requests, costs, streams=2):
+def served_mix(n_requests, costs, streams=3):
     """Poisson mix of Q4/T3/Hex20 requests (1e3-1e6 elements, 60% mesh reuse).
     The arrival rate puts the offered load at 75% of one stream's measured
     service rate: a separate sample of 200 requests from the same
@@ -542,7 +542,7 @@ def main():
     ap.add_argument("--extras", default="coloring,autotune,batch,c3,hex20,serve",
                     help="comma list of secondary measurements to run (or 'none')")
     ap.add_argument("--serve-requests", type=int, default=2000)
-    ap.add_argument("--serve-streams", type=int, default=2, help="server workers (the last is the express lane)")
+    ap.add_argument("--serve-streams", type=int, default=3, help="server workers (the last is the express lane)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="end with one device-to-device copy of BYTES (a copy kernel of known size, the "
                          "scale for rocprofv3 FETCH_SIZE / WRITE_SIZE)")

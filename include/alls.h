@@ -274,10 +274,9 @@ typedef enum { ALLS_CACHE_COST = 0, ALLS_CACHE_EAGER = 1 } alls_cache_policy;
 
 typedef struct {
     int32_t device;
-    int32_t n_streams;        /* concurrent in-flight requests; 0 => 2.  With 2 or more the last
+    int32_t n_streams;        /* concurrent in-flight requests; 0 => 3.  With 2 or more the last
                                  is an express lane for small requests; the others take the queue
-                                 in order, one heavy request (over about 8 ms) at a time.  3 gives
-                                 lower latency, 2 keeps more patterns cached */
+                                 in order, one heavy request (over about 8 ms) at a time */
     int64_t hbm_budget_bytes; /* admission budget; 0 => 55% of free HBM, measured after the
                                  process's idle cached blocks are returned to the device */
     int32_t pattern_cache_entries;
